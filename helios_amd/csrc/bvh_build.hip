@@ -1,0 +1,432 @@
+// bvh_build.hip — GPU construction of the wide BVHs (one BLAS per mesh, one
+// TLAS over mesh-node instances).
+//
+// Replaces the driver's acceleration-structure builds
+// (src/engine/resource/mesh.cpp:46-117: one BLAS per mesh, one geometry per
+// submesh; src/engine/gfx/renderer.cpp:113-181: TLAS over <= 1024 instances).
+//
+// Method: linear BVH over 30-bit Morton codes (Karras, "Maximizing Parallelism
+// in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012): sort,
+// radix tree, bottom-up bounds with arrival counters; then a top-down collapse
+// of the binary tree into W-wide nodes (W = 4 or 8) that repeatedly opens the
+// child with the largest surface area, keeping subtrees of <= leaf_size
+// primitives as leaves (each LBVH subtree covers a contiguous range of the
+// sorted primitives, so a leaf is a range).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "gpu_scene.hpp"
+#include "internal.hpp"
+
+namespace hh {
+
+namespace {
+
+__device__ inline uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// float <-> order-preserving uint for atomic min/max
+__device__ inline uint32_t f2o(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float o2f(uint32_t u) {
+    uint32_t v = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    float f;
+    __builtin_memcpy(&f, &v, 4);
+    return f;
+}
+
+__global__ void k_tri_aabbs(const float4* __restrict__ tri, uint32_t n, float4* __restrict__ lo, float4* __restrict__ hi,
+                            uint32_t* __restrict__ cbounds) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float c[3] = {0, 0, 0};
+    bool ok = i < n;
+    if (ok) {
+        float4 a = tri[3 * i], b = tri[3 * i + 1], d = tri[3 * i + 2];
+        float4 l = make_float4(fminf(a.x, fminf(b.x, d.x)), fminf(a.y, fminf(b.y, d.y)), fminf(a.z, fminf(b.z, d.z)), 0.f);
+        float4 h = make_float4(fmaxf(a.x, fmaxf(b.x, d.x)), fmaxf(a.y, fmaxf(b.y, d.y)), fmaxf(a.z, fmaxf(b.z, d.z)), 0.f);
+        lo[i] = l;
+        hi[i] = h;
+        c[0] = 0.5f * (l.x + h.x);
+        c[1] = 0.5f * (l.y + h.y);
+        c[2] = 0.5f * (l.z + h.z);
+    }
+    // wave-level min/max, then one atomic per wave
+    for (int k = 0; k < 3; k++) {
+        float mn = ok ? c[k] : INFINITY, mx = ok ? c[k] : -INFINITY;
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off));
+            mx = fmaxf(mx, __shfl_xor(mx, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&cbounds[k], f2o(mn));
+            atomicMax(&cbounds[3 + k], f2o(mx));
+        }
+    }
+}
+
+__global__ void k_box_centroids(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t n,
+                                uint32_t* __restrict__ cbounds) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = i < n;
+    float c[3] = {0, 0, 0};
+    if (ok) {
+        float4 l = lo[i], h = hi[i];
+        c[0] = 0.5f * (l.x + h.x);
+        c[1] = 0.5f * (l.y + h.y);
+        c[2] = 0.5f * (l.z + h.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        float mn = ok ? c[k] : INFINITY, mx = ok ? c[k] : -INFINITY;
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off));
+            mx = fmaxf(mx, __shfl_xor(mx, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&cbounds[k], f2o(mn));
+            atomicMax(&cbounds[3 + k], f2o(mx));
+        }
+    }
+}
+
+__global__ void k_init_bounds(uint32_t* cb) {
+    if (threadIdx.x < 3) {
+        cb[threadIdx.x] = 0xFFFFFFFFu;
+        cb[3 + threadIdx.x] = 0u;
+    }
+}
+
+__global__ void k_morton(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t n,
+                         const uint32_t* __restrict__ cbounds, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 l = lo[i], h = hi[i];
+    float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float mn = o2f(cbounds[k]), mx = o2f(cbounds[3 + k]);
+        float ext = mx - mn;
+        float t = ext > 0.f ? (c[k] - mn) / ext : 0.5f;
+        q[k] = (uint32_t)fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+    }
+    keys[i] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    vals[i] = i;
+}
+
+__device__ inline int delta(const uint32_t* keys, int n, int i, int j) {
+    if (j < 0 || j > n - 1) return -1;
+    uint32_t a = keys[i], b = keys[j];
+    if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+    return __clz(a ^ b);
+}
+
+// Karras 2012, Fig. 4. Internal nodes 0..n-2; leaf j has binary ref n-1+j.
+__global__ void k_radix_tree(const uint32_t* __restrict__ keys, int n, int2* __restrict__ childs,
+                             uint32_t* __restrict__ parent, int2* __restrict__ range) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int lo_ = min(i, j), hi_ = max(i, j);
+    int left = (lo_ == gamma) ? (n - 1 + gamma) : gamma;
+    int right = (hi_ == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    childs[i] = make_int2(left, right);
+    range[i] = make_int2(lo_, hi_);
+    parent[left] = (uint32_t)i;
+    parent[right] = (uint32_t)i;
+}
+
+__device__ inline float4 aload4(const float4* p) {
+    const uint32_t* q = (const uint32_t*)p;
+    float4 r;
+    r.x = __uint_as_float(__hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    r.y = __uint_as_float(__hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    r.z = __uint_as_float(__hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    r.w = __uint_as_float(__hip_atomic_load(q + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    return r;
+}
+
+// Bottom-up bounds: node boxes live in blo/bhi indexed by binary ref
+// (internal 0..n-2, leaves n-1..2n-2); .w of blo holds the primitive count.
+__global__ void k_refit(const float4* __restrict__ plo, const float4* __restrict__ phi, const uint32_t* __restrict__ order,
+                        int n, const int2* __restrict__ childs, const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags,
+                        float4* blo, float4* bhi) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t p = order[j];
+    float4 l = plo[p], h = phi[p];
+    l.w = 1.0f;
+    blo[n - 1 + j] = l;
+    bhi[n - 1 + j] = h;
+    if (n == 1) return;
+    uint32_t node = parent[n - 1 + j];
+    for (;;) {
+        __threadfence();
+        uint32_t old = atomicAdd(&flags[node], 1u);
+        if (old == 0) return;
+        __threadfence();
+        int2 c = childs[node];
+        float4 al = aload4(&blo[c.x]), ah = aload4(&bhi[c.x]);
+        float4 bl = aload4(&blo[c.y]), bh = aload4(&bhi[c.y]);
+        float4 rl = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), al.w + bl.w);
+        float4 rh = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+        uint32_t* ql = (uint32_t*)&blo[node];
+        uint32_t* qh = (uint32_t*)&bhi[node];
+        __hip_atomic_store(ql + 0, __float_as_uint(rl.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ql + 1, __float_as_uint(rl.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ql + 2, __float_as_uint(rl.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ql + 3, __float_as_uint(rl.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(qh + 0, __float_as_uint(rh.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(qh + 1, __float_as_uint(rh.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(qh + 2, __float_as_uint(rh.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(qh + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (node == 0) return;
+        node = parent[node];
+    }
+}
+
+__device__ inline float half_area(float4 l, float4 h) {
+    float dx = h.x - l.x, dy = h.y - l.y, dz = h.z - l.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// One wide node per job: job = (binary ref, wide index).
+template <int W>
+__global__ void k_collapse(const int2* __restrict__ jobs, const uint32_t* __restrict__ njobs_p, int2* __restrict__ out_jobs,
+                           uint32_t* __restrict__ out_njobs, uint32_t* __restrict__ node_counter, int n, int leaf_size,
+                           const int2* __restrict__ childs, const int2* __restrict__ range, const float4* __restrict__ blo,
+                           const float4* __restrict__ bhi, const uint32_t* __restrict__ order, WideNode<W>* __restrict__ nodes,
+                           uint32_t node_base, uint32_t leaf_base, int leaf_is_prim) {
+    uint32_t jid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jid >= *njobs_p) return;
+    int2 job = jobs[jid];
+    int bref = job.x;
+    auto is_internal = [&](int r) { return r < n - 1; };
+    auto count_of = [&](int r) { return is_internal(r) ? (int)blo[r].w : 1; };
+    auto expandable = [&](int r) { return is_internal(r) && count_of(r) > leaf_size; };
+    int c[W];
+    int nc = 0;
+    if (!expandable(bref)) {
+        c[nc++] = bref;
+    } else {
+        c[nc++] = childs[bref].x;
+        c[nc++] = childs[bref].y;
+        while (nc < W) {
+            int best = -1;
+            float ba = -1.f;
+            for (int k = 0; k < nc; k++)
+                if (expandable(c[k])) {
+                    float a = half_area(blo[c[k]], bhi[c[k]]);
+                    if (a > ba) { ba = a; best = k; }
+                }
+            if (best < 0) break;
+            int r = c[best];
+            c[best] = childs[r].x;
+            c[nc++] = childs[r].y;
+        }
+    }
+    WideNode<W> nd;
+    for (int k = 0; k < W; k++) {
+        nd.lox[k] = nd.loy[k] = nd.loz[k] = 0.f;
+        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.f;
+        nd.child[k] = 0;
+    }
+    nd.nchild = (uint32_t)nc;
+    for (int k = 0; k < W - 1; k++) nd.pad[k] = 0;
+    for (int k = 0; k < nc; k++) {
+        int r = c[k];
+        float4 l = blo[r], h = bhi[r];
+        nd.lox[k] = l.x; nd.loy[k] = l.y; nd.loz[k] = l.z;
+        nd.hix[k] = h.x; nd.hiy[k] = h.y; nd.hiz[k] = h.z;
+        if (expandable(r)) {
+            uint32_t idx = atomicAdd(node_counter, 1u);
+            uint32_t slot = atomicAdd(out_njobs, 1u);
+            out_jobs[slot] = make_int2(r, (int)idx);
+            nd.child[k] = node_base + idx;
+        } else {
+            int first, cnt;
+            if (is_internal(r)) { first = range[r].x; cnt = range[r].y - range[r].x + 1; }
+            else { first = r - (n - 1); cnt = 1; }
+            nd.child[k] = leaf_is_prim ? make_leaf(order[first], 1) : make_leaf(leaf_base + (uint32_t)first, (uint32_t)cnt);
+        }
+    }
+    nodes[job.y] = nd;
+}
+
+__global__ void k_gather_tris(const float4* __restrict__ src, const uint32_t* __restrict__ order, uint32_t n,
+                              float4* __restrict__ dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = order[i];
+    dst[3 * i] = src[3 * p];
+    dst[3 * i + 1] = src[3 * p + 1];
+    dst[3 * i + 2] = src[3 * p + 2];
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+// Builds a W-wide BVH over n primitive AABBs (device lo/hi float4 arrays).
+// Nodes are written to `nodes` starting at index 0 of that array; child
+// references are offset by node_base (internal) and leaf_base (leaves).
+// On return *order_out (device, n entries) holds the primitive index of every
+// leaf slot and root_box the bounds of all primitives.
+template <int W>
+static hipError_t build_wide(hipStream_t st, uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_cbounds,
+                             WideNode<W>* d_nodes, uint32_t node_base, uint32_t leaf_base, int leaf_size, int leaf_is_prim,
+                             uint32_t* d_order, BuildScratch& sc, uint32_t* num_nodes, float root_box[6]) {
+    HH_TRY(sc.ensure(n));
+    const uint32_t B = 256;
+    k_morton<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, n, d_cbounds, sc.keys_in, sc.vals_in);
+    size_t tb = sc.sort_bytes;
+    HH_TRY(hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.keys_in, sc.keys_out, sc.vals_in, d_order, (int)n, 0, 30, st));
+    if (n > 1) {
+        k_radix_tree<<<blocks(n - 1, B), B, 0, st>>>(sc.keys_out, (int)n, sc.childs, sc.parent, sc.range);
+        k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
+    }
+    k_refit<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, d_order, (int)n, sc.childs, sc.parent, sc.flags, sc.blo, sc.bhi);
+    int root = n > 1 ? 0 : 0;  // binary ref of the root (leaf 0 when n == 1: ref n-1+0 = 0)
+    int2 job0 = make_int2(root, 0);
+    uint32_t one = 1;
+    HH_TRY(hipMemcpyAsync(sc.jobs_a, &job0, sizeof(int2), hipMemcpyHostToDevice, st));
+    HH_TRY(hipMemcpyAsync(sc.counters, &one, 4, hipMemcpyHostToDevice, st));       // jobs in A
+    HH_TRY(hipMemcpyAsync(sc.counters + 2, &one, 4, hipMemcpyHostToDevice, st));   // node counter
+    uint32_t njobs = 1;
+    int2* qa = sc.jobs_a;
+    int2* qb = sc.jobs_b;
+    uint32_t* ca = sc.counters;
+    uint32_t* cb = sc.counters + 1;
+    while (njobs > 0) {
+        k_fill_u32<<<1, 1, 0, st>>>(cb, 0u, 1);
+        k_collapse<W><<<blocks(njobs, 64), 64, 0, st>>>(qa, ca, qb, cb, sc.counters + 2, (int)n, leaf_size, sc.childs, sc.range,
+                                                         sc.blo, sc.bhi, d_order, d_nodes, node_base, leaf_base, leaf_is_prim);
+        HH_TRY(hipMemcpyAsync(&njobs, cb, 4, hipMemcpyDeviceToHost, st));
+        HH_TRY(hipStreamSynchronize(st));
+        std::swap(qa, qb);
+        std::swap(ca, cb);
+    }
+    HH_TRY(hipMemcpyAsync(num_nodes, sc.counters + 2, 4, hipMemcpyDeviceToHost, st));
+    float4 rb[2];
+    HH_TRY(hipMemcpyAsync(&rb[0], &sc.blo[0], 16, hipMemcpyDeviceToHost, st));
+    HH_TRY(hipMemcpyAsync(&rb[1], &sc.bhi[0], 16, hipMemcpyDeviceToHost, st));
+    HH_TRY(hipStreamSynchronize(st));
+    root_box[0] = rb[0].x; root_box[1] = rb[0].y; root_box[2] = rb[0].z;
+    root_box[3] = rb[1].x; root_box[4] = rb[1].y; root_box[5] = rb[1].z;
+    return hipSuccess;
+}
+
+hipError_t BuildScratch::ensure(uint32_t n) {
+    if (n <= cap) return hipSuccess;
+    release();
+    cap = std::max<uint32_t>(n, 1024);
+    size_t N = cap;
+    HH_TRY(hipMalloc(&keys_in, N * 4));
+    HH_TRY(hipMalloc(&keys_out, N * 4));
+    HH_TRY(hipMalloc(&vals_in, N * 4));
+    HH_TRY(hipMalloc(&order, N * 4));
+    HH_TRY(hipMalloc(&childs, N * sizeof(int2)));
+    HH_TRY(hipMalloc(&range, N * sizeof(int2)));
+    HH_TRY(hipMalloc(&parent, 2 * N * 4));
+    HH_TRY(hipMalloc(&flags, N * 4));
+    HH_TRY(hipMalloc(&blo, 2 * N * sizeof(float4)));
+    HH_TRY(hipMalloc(&bhi, 2 * N * sizeof(float4)));
+    HH_TRY(hipMalloc(&jobs_a, N * sizeof(int2)));
+    HH_TRY(hipMalloc(&jobs_b, N * sizeof(int2)));
+    HH_TRY(hipMalloc(&counters, 16));
+    HH_TRY(hipMalloc(&cbounds, 32));
+    HH_TRY(hipMalloc(&aabb_lo, N * sizeof(float4)));
+    HH_TRY(hipMalloc(&aabb_hi, N * sizeof(float4)));
+    sort_bytes = 0;
+    HH_TRY(hipcub::DeviceRadixSort::SortPairs((void*)nullptr, sort_bytes, keys_in, keys_out, vals_in, vals_in, (int)N, 0, 30));
+    HH_TRY(hipMalloc(&sort_tmp, sort_bytes + 256));
+    return hipSuccess;
+}
+
+void BuildScratch::release() {
+    void* ps[] = {keys_in, keys_out, vals_in, order, childs, range, parent, flags, blo, bhi, jobs_a, jobs_b, counters, cbounds,
+                  aabb_lo, aabb_hi, sort_tmp};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    keys_in = keys_out = vals_in = order = parent = flags = counters = cbounds = nullptr;
+    childs = range = jobs_a = jobs_b = nullptr;
+    blo = bhi = aabb_lo = aabb_hi = nullptr;
+    sort_tmp = nullptr;
+    cap = 0;
+}
+
+// BLAS over a triangle soup (3 float4 per triangle, see gpu_scene.hpp):
+// writes the leaf-ordered triangles to d_tri_pool + tri_base and the nodes to
+// d_node_pool + node_base (global indices in child references).
+hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
+                      uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
+                      float root_box[6]) {
+    HH_TRY(sc.ensure(n));
+    const uint32_t B = 256;
+    k_init_bounds<<<1, 64, 0, st>>>(sc.cbounds);
+    k_tri_aabbs<<<blocks(n, B), B, 0, st>>>(d_tri_src, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds);
+    uint32_t* order = sc.vals_out_tmp(n);
+    hipError_t e;
+    if (width == 8)
+        e = build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base,
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box);
+    else
+        e = build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base,
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box);
+    HH_TRY(e);
+    k_gather_tris<<<blocks(n, B), B, 0, st>>>(d_tri_src, order, n, d_tri_pool + 3 * (size_t)tri_base);
+    return hipGetLastError();
+}
+
+// TLAS over instance AABBs (host arrays, n >= 1); leaves hold one instance.
+hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes) {
+    HH_TRY(sc.ensure(n));
+    std::vector<float4> lo(n), hi(n);
+    for (uint32_t i = 0; i < n; i++) {
+        lo[i] = make_float4(h_lo[3 * i], h_lo[3 * i + 1], h_lo[3 * i + 2], 0.f);
+        hi[i] = make_float4(h_hi[3 * i], h_hi[3 * i + 1], h_hi[3 * i + 2], 0.f);
+    }
+    HH_TRY(hipMemcpyAsync(sc.aabb_lo, lo.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+    HH_TRY(hipMemcpyAsync(sc.aabb_hi, hi.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+    const uint32_t B = 256;
+    k_init_bounds<<<1, 64, 0, st>>>(sc.cbounds);
+    k_box_centroids<<<blocks(n, B), B, 0, st>>>(sc.aabb_lo, sc.aabb_hi, n, sc.cbounds);
+    uint32_t* order = sc.vals_out_tmp(n);
+    float rb[6];
+    if (width == 8)
+        return build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base, 0, 1,
+                             1, order, sc, num_nodes, rb);
+    return build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base, 0, 1, 1,
+                         order, sc, num_nodes, rb);
+}
+
+}  // namespace hh

@@ -1,0 +1,734 @@
+// integrator.hip — the path integrator as wavefront kernels for gfx950.
+//
+// Replaces the Vulkan ray-tracing pipeline of Helios (raygen
+// path_trace_rgen.glsl:132-250, closest-hit path_trace_rchit.glsl:136-580,
+// any-hit path_trace_rahit.glsl:174-188, miss path_trace_rmiss.glsl:38-67,
+// shadow hit/miss path_trace_shadow.{rchit,rmiss}:16-19) launched once per
+// frame by PathIntegrator::launch_rays (src/engine/gfx/path_integrator.cpp:199).
+//
+// Per frame, for every path slot of every view of a batch:
+//   generate   : RNG init + camera ray (4 draws)        -> extension queue 0
+//   for depth d: extend  (closest hit, any-hit alpha only at d = 0)
+//                shade   (emission, NEE light sample, uber BRDF bounce, RR)
+//                        -> shadow queue, extension queue d+1 (compacted with
+//                           wave ballots: one atomic per wave)
+//                shadow  (visibility; adds the NEE contribution if unoccluded)
+//   accumulate : min(L, 1) + progressive mean into the RGBA32F surface.
+// Every kernel is persistent (grid = CUs x blocks_per_cu, grid-stride over the
+// queue length read from device memory), so the launch sequence of a frame is
+// fixed and is captured once as a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "gpu_common.hpp"
+#include "gpu_scene.hpp"
+#include "internal.hpp"
+#include "traverse.hpp"
+
+namespace hh {
+
+// ---------------------------------------------------------------------------
+// Shared device helpers
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated queue allocation: one atomic per wave.
+__device__ inline uint32_t wave_alloc(uint32_t* counter, bool want) {
+    unsigned long long m = __ballot(want);
+    if (m == 0ull) return 0u;
+    uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    unsigned long long below = (lane_id() == 0) ? 0ull : (m & ((1ull << lane_id()) - 1ull));
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ inline uint32_t find_view(const ViewGPU* views, uint32_t nv, uint32_t slot) {
+    uint32_t lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (views[mid].slot_base <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct SlotInfo {
+    uint32_t view, x, y, sample;
+};
+__device__ inline SlotInfo slot_info(const ViewGPU* views, uint32_t nv, uint32_t slot) {
+    SlotInfo si;
+    si.view = nv == 1 ? 0u : find_view(views, nv, slot);
+    const ViewGPU& v = views[si.view];
+    uint32_t local = slot - v.slot_base;
+    uint32_t npix = v.tile_w * v.tile_h;
+    si.sample = local / npix;
+    uint32_t p = local - si.sample * npix;
+    si.y = v.tile_y + p / v.tile_w;
+    si.x = v.tile_x + p % v.tile_w;
+    return si;
+}
+
+// generate_ray (path_trace_rgen.glsl:132-174); operation order fixed to match
+// the CPU restatement bit for bit.
+__device__ inline void generate_ray(const ViewGPU& v, uint32_t px, uint32_t py, RNG& rng, f3& o, f3& d) {
+    float jx = next_float(rng);
+    float jy = next_float(rng);
+    float tcx = (((float)px + 0.5f) + jx) / (float)v.width;
+    float tcy = (((float)py + 0.5f) + jy) / (float)v.height;
+    float nx = tcx * 2.0f - 1.0f, ny = tcy * 2.0f - 1.0f;
+    float tg[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) tg[r] = ((v.vpi[0 * 4 + r] * nx + v.vpi[1 * 4 + r] * ny) + v.vpi[2 * 4 + r] * 0.0f) + v.vpi[3 * 4 + r] * 1.0f;
+    f3 target = mk(tg[0] / tg[3], tg[1] / tg[3], tg[2] / tg[3]);
+    float angle = next_float(rng) * 2.0f * HH_PI;
+    float radius = sqrtf(next_float(rng));
+    float ox = cosf(angle) * radius * v.aperture, oy = sinf(angle) * radius * v.aperture;
+    f3 cam = mk(v.cam[0], v.cam[1], v.cam[2]);
+    f3 ap = (cam + mk(v.right[0], v.right[1], v.right[2]) * ox) + mk(v.up[0], v.up[1], v.up[2]) * oy;
+    f3 rdir = -normalize(target - cam);
+    f3 fp = mk(v.fplane[0], v.fplane[1], v.fplane[2]);
+    float t = -(dot(cam, fp) + v.fplane[3]) / dot(rdir, fp);
+    f3 focus = cam + rdir * t;
+    o = ap;
+    d = normalize(focus - ap);
+}
+
+// Cube-map sampling: Vulkan face selection, bilinear within the face.
+__device__ inline f3 sample_env(const SceneGPU& S, f3 d) {
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x >= 0.0f) { face = 0; sc = -d.z; tc = -d.y; } else { face = 1; sc = d.z; tc = -d.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d.y >= 0.0f) { face = 2; sc = d.x; tc = d.z; } else { face = 3; sc = d.x; tc = -d.z; }
+    } else {
+        ma = az;
+        if (d.z >= 0.0f) { face = 4; sc = d.x; tc = -d.y; } else { face = 5; sc = -d.x; tc = -d.y; }
+    }
+    if (!(ma > 0.0f)) return mk(0.f, 0.f, 0.f);
+    const int n = (int)S.env_size;
+    float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
+    float x = s * (float)n - 0.5f, y = t * (float)n - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float wx = x - fx, wy = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    int i1 = min(max(i0 + 1, 0), n - 1), j1 = min(max(j0 + 1, 0), n - 1);
+    i0 = min(max(i0, 0), n - 1);
+    j0 = min(max(j0, 0), n - 1);
+    const float4* base = S.env + (size_t)face * n * n;
+    float4 p00 = base[j0 * n + i0], p10 = base[j0 * n + i1], p01 = base[j1 * n + i0], p11 = base[j1 * n + i1];
+    return mk(lerpf(lerpf(p00.x, p10.x, wx), lerpf(p01.x, p11.x, wx), wy),
+              lerpf(lerpf(p00.y, p10.y, wx), lerpf(p01.y, p11.y, wx), wy),
+              lerpf(lerpf(p00.z, p10.z, wx), lerpf(p01.z, p11.z, wx), wy));
+}
+
+__device__ inline f3 m4_point(const float* m, f3 v, float w) {
+    return mk(m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * w, m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * w,
+              m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14] * w);
+}
+__device__ inline f3 m3_mul(const float* m, f3 v) {
+    return mk(m[0] * v.x + m[4] * v.y + m[8] * v.z, m[1] * v.x + m[5] * v.y + m[9] * v.z, m[2] * v.x + m[6] * v.y + m[10] * v.z);
+}
+
+// populate_surface_properties (path_trace_rchit.glsl:256-280)
+__device__ inline void populate(const SceneGPU& S, uint32_t inst, uint32_t tri, float u, float v, Surface& p) {
+    const float4* tw = S.tris + 3 * (size_t)tri;
+    uint32_t prim = __float_as_uint(tw[0].w);
+    uint32_t geom = __float_as_uint(tw[1].w) & 0x7FFFFFFFu;
+    const InstanceData& I = S.inst[inst];
+    uint32_t mesh = S.inst_trav[inst].mesh;
+    uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
+    const MaterialData& m = S.mats[info.y];
+    uint32_t pid = prim + info.x;
+    const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * pid;
+    const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
+    const float4* V0 = vb + 5 * (size_t)ix[0];
+    const float4* V1 = vb + 5 * (size_t)ix[1];
+    const float4* V2 = vb + 5 * (size_t)ix[2];
+    float b0 = 1.0f - u - v, b1 = u, b2 = v;
+    f3 pos = ld3(V0[0]) * b0 + ld3(V1[0]) * b1 + ld3(V2[0]) * b2;
+    float4 t0 = V0[1], t1 = V1[1], t2 = V2[1];
+    float uv0 = t0.x * b0 + t1.x * b1 + t2.x * b2;
+    float uv1 = t0.y * b0 + t1.y * b1 + t2.y * b2;
+    f3 n = normalize(ld3(V0[2]) * b0 + ld3(V1[2]) * b1 + ld3(V2[2]) * b2);
+    p.pos = m4_point(I.model, pos, 1.0f);
+    p.vnormal = m3_mul(I.normal, n);
+    float4 alb;
+    if (m.tex0[0] == -1) alb = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], m.albedo[3]);
+    else alb = sample_tex(S, m.tex0[0], uv0, uv1);
+    p.albedo = mk(alb.x, alb.y, alb.z);
+    p.alpha_a = alb.w;
+    if (m.tex0[1] == -1) {
+        p.normal = p.vnormal;
+    } else {
+        f3 tg = normalize(ld3(V0[3]) * b0 + ld3(V1[3]) * b1 + ld3(V2[3]) * b2);
+        f3 bt = normalize(ld3(V0[4]) * b0 + ld3(V1[4]) * b1 + ld3(V2[4]) * b2);
+        f3 T = normalize(m3_mul(I.normal, tg)), B = normalize(m3_mul(I.normal, bt)), N = normalize(p.vnormal);
+        float4 c = sample_tex(S, m.tex0[1], uv0, uv1);
+        f3 tn = normalize(mk(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f));
+        p.normal = normalize(T * tn.x + B * tn.y + N * tn.z);
+    }
+    auto chan = [](float4 c, int k) { k &= 3; return k == 0 ? c.x : (k == 1 ? c.y : (k == 2 ? c.z : c.w)); };
+    p.roughness = m.tex0[2] == -1 ? m.roughness_metallic[0] : chan(sample_tex(S, m.tex0[2], uv0, uv1), m.tex1[2]);
+    p.metallic = m.tex0[3] == -1 ? m.roughness_metallic[1] : chan(sample_tex(S, m.tex0[3], uv0, uv1), m.tex1[3]);
+    if (m.tex1[0] == -1) p.emissive = mk(m.emissive[0], m.emissive[1], m.emissive[2]);
+    else {
+        float4 c = sample_tex(S, m.tex1[0], uv0, uv1);
+        p.emissive = mk(c.x, c.y, c.z);
+    }
+    p.roughness = fmaxf(p.roughness, HH_MIN_ROUGHNESS);
+    p.F0 = mk(0.03f, 0.03f, 0.03f) * (1.0f - p.metallic) + p.albedo * p.metallic;
+}
+
+// One NEE light sample (path_trace_rchit.glsl:284-451, 455-483). Returns the
+// contribution that becomes radiance if the shadow ray is unoccluded, and the
+// shadow ray. want_ray = false when the contribution is exactly zero.
+struct ShadowRay {
+    f3 o, d;
+    float tmin, tmax;
+    f3 contrib;
+    uint32_t flags;
+    bool want;
+};
+__device__ inline ShadowRay direct_lighting(const SceneGPU& S, const Surface& p, f3 ray_dir, f3 T, uint32_t depth,
+                                            float shadow_bias, RNG& rng) {
+    ShadowRay sr;
+    sr.want = false;
+    sr.flags = depth == 0 ? 0u : (HH_TRACE_OPAQUE_BIT | HH_TRACE_TERMINATE_BIT);
+    sr.tmin = 0.0001f;
+    sr.tmax = 10000.0f;
+    sr.o = p.pos + p.normal * shadow_bias;
+    uint32_t nl = S.num_lights;
+    uint32_t li = next_uint(rng, nl);
+    if (nl == 0) return sr;
+    const LightData& L = S.lights[li];
+    uint32_t type = (uint32_t)L.d0[0];
+    f3 Wi = mk(0.f, 0.f, 0.f), Li = mk(0.f, 0.f, 0.f);
+    float pdf = 0.0f;
+    if (type == 0u) {
+        float r0 = next_float(rng);
+        float r1 = next_float(rng);
+        f3 ld = -mk(L.d1[0], L.d1[1], L.d1[2]);
+        f3 lt = normalize(cross(ld, mk(0.f, 1.f, 0.f)));
+        f3 lb = normalize(cross(lt, ld));
+        float pr = L.d2[3] * sqrtf(r0), pa = r1 * 2.0f * HH_PI;
+        Wi = normalize(ld + lt * (pr * cosf(pa)) + lb * (pr * sinf(pa)));
+        Li = mk(L.d0[1], L.d0[2], L.d0[3]) * L.d1[3];
+    } else if (type == 1u || type == 2u) {
+        float r0 = next_float(rng);
+        float r1 = next_float(rng);
+        f3 to = mk(L.d2[0], L.d2[1], L.d2[2]) - p.pos;
+        f3 ld = normalize(to);
+        float dist = length(to);
+        float lr = L.d2[3] / dist;
+        f3 lt = normalize(cross(ld, mk(0.f, 1.f, 0.f)));
+        f3 lb = normalize(cross(lt, ld));
+        float pr = lr * sqrtf(r0), pa = r1 * 2.0f * HH_PI;
+        Wi = normalize(ld + lt * (pr * cosf(pa)) + lb * (pr * sinf(pa)));
+        if (type == 1u) {
+            float x = dot(ld, -mk(L.d1[0], L.d1[1], L.d1[2]));
+            float tt = clampf((x - L.d3[1]) / (L.d3[0] - L.d3[1]), 0.0f, 1.0f);
+            float att = tt * tt * (3.0f - 2.0f * tt);
+            Li = mk(L.d0[1], L.d0[2], L.d0[3]) * L.d1[3] * att / (dist * dist);
+        } else {
+            Li = mk(L.d0[1], L.d0[2], L.d0[3]) * L.d1[3] / (dist * dist);
+        }
+        sr.tmax = dist;
+    } else if (type == 3u) {
+        float r0 = next_float(rng);
+        float r1 = next_float(rng);
+        Wi = sample_cosine_lobe(p.normal, r0, r1);
+        Li = sample_env(S, Wi);
+        pdf = dot(p.normal, Wi) / HH_PI;
+    } else if (type == 4u) {
+        uint32_t mesh_node = (uint32_t)L.d0[1];
+        uint32_t prim = next_uint(rng, (uint32_t)L.d1[2]);  // reference reads .z (= 0)
+        uint32_t mat_idx = (uint32_t)L.d0[2], off = (uint32_t)L.d0[3];
+        const InstanceData& I = S.inst[mesh_node];
+        uint32_t mesh = S.inst_trav[mesh_node].mesh;
+        uint32_t pid = prim + off;
+        const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * pid;
+        const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
+        float b0 = next_float(rng);
+        float b1 = next_float(rng);
+        float su0 = sqrtf(b0);
+        float bx = 1.0f - su0, by = b1 * su0;
+        float4 q0 = vb[5 * (size_t)ix[0]], q1 = vb[5 * (size_t)ix[1]], q2 = vb[5 * (size_t)ix[2]];
+        f3 P0 = m4_point(I.model, ld3(q0), q0.w), P1 = m4_point(I.model, ld3(q1), q1.w), P2 = m4_point(I.model, ld3(q2), q2.w);
+        float w0 = 1.0f - bx - by;
+        f3 lp = P0 * w0 + P1 * bx + P2 * by;
+        f3 ln = normalize(m3_mul(I.normal, ld3(vb[5 * (size_t)ix[0] + 2]) * w0 + ld3(vb[5 * (size_t)ix[1] + 2]) * bx +
+                                                ld3(vb[5 * (size_t)ix[2] + 2]) * by));
+        f3 ldir = p.pos - lp;
+        float dsq = dot(ldir, ldir);
+        float area = 0.5f * length(cross(P1 - P0, P2 - P0));
+        if (area == 0.0f || dsq == 0.0f) return sr;
+        float dist = sqrtf(dsq);
+        ldir = ldir / dist;
+        sr.tmax = fmaxf(0.0f, dist - HH_EPS);
+        float ct = dot(ln, ldir);
+        if (ct == 0.0f) return sr;
+        const MaterialData& lm = S.mats[mat_idx];
+        Li = mk(lm.emissive[0], lm.emissive[1], lm.emissive[2]);
+        Wi = -ldir;
+        pdf = dsq / fmaxf(HH_EPS, ct * area);
+    }
+    f3 Wo = -ray_dir;
+    f3 Wh = normalize(Wo + Wi);
+    f3 brdf = evaluate_uber(p, Wo, Wh, Wi);
+    float ct = clampf(dot(p.normal, Wi), 0.0f, 1.0f);
+    if (is_black(Li)) return sr;
+    f3 c = T * brdf * ct * Li;
+    if (pdf != 0.0f) c = c / pdf;
+    c = c * (float)nl;
+    sr.contrib = c;
+    sr.d = Wi;
+    sr.want = !is_black(c);
+    return sr;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ views, uint32_t nv, uint32_t total,
+                                                 float4* __restrict__ q_o, float4* __restrict__ q_d,
+                                                 uint32_t* __restrict__ q_slot, float4* __restrict__ T, float4* __restrict__ L,
+                                                 uint2* __restrict__ rngs, uint32_t* __restrict__ counters) {
+    const uint32_t stride = gridDim.x * BS;
+    for (uint32_t slot = blockIdx.x * BS + threadIdx.x; slot < total; slot += stride) {
+        SlotInfo si = slot_info(views, nv, slot);
+        const ViewGPU& v = views[si.view];
+        RNG rng = rng_init(si.x, si.y, v.first_frame + si.sample);
+        f3 o, d;
+        generate_ray(v, si.x, si.y, rng, o, d);
+        q_o[slot] = make_float4(o.x, o.y, o.z, 0.001f);
+        q_d[slot] = make_float4(d.x, d.y, d.z, 10000.0f);
+        q_slot[slot] = slot;
+        T[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rngs[slot] = make_uint2(rng.s0, rng.s1);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[0] = total;
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+}
+
+template <int W, int BS>
+__global__ void __launch_bounds__(BS) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+                                               const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                                               const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count_p,
+                                               uint32_t rflags, float4* __restrict__ hit, uint32_t* __restrict__ hit_tri,
+                                               unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t stack[kLdsStack * BS];
+    const uint32_t n = *count_p;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)n);
+    const uint32_t stride = gridDim.x * BS;
+    for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n; q += stride) {
+        uint32_t slot = q_slot[q];
+        uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
+        const SceneGPU& S = scenes[views[view].scene];
+        float4 o = q_o[q], d = q_d[q];
+        HitRec h;
+        trace_ray<W, false>(S, ld3(o), ld3(d), o.w, d.w, rflags, h, &stack[threadIdx.x], BS);
+        hit[q] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.inst));
+        hit_tri[q] = h.found ? h.tri : 0xFFFFFFFFu;
+    }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+                                              uint32_t depth, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                                              const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count_p,
+                                              const float4* __restrict__ hit, const uint32_t* __restrict__ hit_tri,
+                                              float4* __restrict__ T, float4* __restrict__ L, uint2* __restrict__ rngs,
+                                              float4* __restrict__ n_o, float4* __restrict__ n_d, uint32_t* __restrict__ n_slot,
+                                              uint32_t* __restrict__ n_count, float4* __restrict__ s_o, float4* __restrict__ s_d,
+                                              float4* __restrict__ s_c, uint32_t* __restrict__ s_slot,
+                                              uint32_t* __restrict__ s_count) {
+    const uint32_t n = *count_p;
+    const uint32_t stride = gridDim.x * BS;
+    // Keep whole waves in the loop so the ballot-based compaction sees every
+    // active lane: iterate over the count rounded up to the grid stride.
+    const uint32_t n_round = ((n + stride - 1) / stride) * stride;
+    for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n_round; q += stride) {
+        const bool active = q < n;
+        bool want_ext = false, want_sh = false;
+        f3 ext_o = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 1.f);
+        ShadowRay sr;
+        sr.want = false;
+        uint32_t slot = 0;
+        if (active) {
+            slot = q_slot[q];
+            uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
+            const ViewGPU& V = views[view];
+            const SceneGPU& S = scenes[V.scene];
+            float4 hh = hit[q];
+            f3 dir = ld3(q_d[q]);
+            float4 T4 = T[slot];
+            f3 Tp = ld3(T4);
+            float4 L4 = L[slot];
+            f3 Lp = ld3(L4);
+            uint2 rs = rngs[slot];
+            RNG rng{rs.x, rs.y};
+            if (hh.x < 0.0f) {
+                // miss (path_trace_rmiss.glsl:60-65)
+                f3 e = sample_env(S, dir);
+                Lp = Lp + (depth == 0 ? e : Tp * e);
+            } else {
+                Surface p;
+                populate(S, __float_as_uint(hh.w), hit_tri[q], hh.y, hh.z, p);
+                if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
+                sr = direct_lighting(S, p, dir, Tp, depth, V.shadow_bias, rng);
+                if ((V.flags & HH_FLAG_DIRECT_ONLY_BIT) == 0u && depth + 1 < V.max_bounces) {
+                    f3 Wo = -dir, Wi;
+                    float pdf;
+                    f3 brdf = sample_uber(p, Wo, rng, Wi, pdf);
+                    float ct = clampf(dot(p.normal, Wi), 0.0f, 1.0f);
+                    f3 Tn = (Tp * (brdf * ct)) / pdf;
+                    float prob = fmaxf(Tn.x, fmaxf(Tn.y, Tn.z));
+                    float rr = next_float(rng);
+                    if (!(rr > prob)) {
+                        Tn = Tn * (1.0f / prob);
+                        want_ext = true;
+                        ext_o = p.pos;
+                        ext_d = Wi;
+                        T[slot] = make_float4(Tn.x, Tn.y, Tn.z, 0.f);
+                    }
+                }
+                rngs[slot] = make_uint2(rng.s0, rng.s1);
+            }
+            L[slot] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
+            want_sh = sr.want;
+        }
+        uint32_t ei = wave_alloc(n_count, want_ext);
+        if (want_ext) {
+            n_o[ei] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0001f);
+            n_d[ei] = make_float4(ext_d.x, ext_d.y, ext_d.z, 10000.0f);
+            n_slot[ei] = slot;
+        }
+        uint32_t si = wave_alloc(s_count, want_sh);
+        if (want_sh) {
+            s_o[si] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmin);
+            s_d[si] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.tmax);
+            s_c[si] = make_float4(sr.contrib.x, sr.contrib.y, sr.contrib.z, __uint_as_float(sr.flags));
+            s_slot[si] = slot;
+        }
+    }
+}
+
+template <int W, int BS>
+__global__ void __launch_bounds__(BS) k_shadow(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+                                               const float4* __restrict__ s_o, const float4* __restrict__ s_d,
+                                               const float4* __restrict__ s_c, const uint32_t* __restrict__ s_slot,
+                                               const uint32_t* __restrict__ count_p, float4* __restrict__ L,
+                                               unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t stack[kLdsStack * BS];
+    const uint32_t n = *count_p;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], (unsigned long long)n);
+    const uint32_t stride = gridDim.x * BS;
+    for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n; q += stride) {
+        uint32_t slot = s_slot[q];
+        uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
+        const SceneGPU& S = scenes[views[view].scene];
+        float4 o = s_o[q], d = s_d[q], c = s_c[q];
+        HitRec h;
+        bool occluded = trace_ray<W, true>(S, ld3(o), ld3(d), o.w, d.w, __float_as_uint(c.w), h, &stack[threadIdx.x], BS);
+        if (!occluded) {
+            float4 l = L[slot];
+            L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+        }
+    }
+}
+
+__global__ void k_reset2(uint32_t* a, uint32_t* b) {
+    if (threadIdx.x == 0) {
+        *a = 0;
+        *b = 0;
+    }
+}
+
+// path_trace_rgen.glsl:219-247: clamp and progressive mean, samples in order.
+__global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, const float4* __restrict__ L) {
+    const ViewGPU& v = views[blockIdx.y];
+    const uint32_t npix = v.tile_w * v.tile_h;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        uint32_t y = v.tile_y + p / v.tile_w, x = v.tile_x + p % v.tile_w;
+        float4* px = v.accum + (size_t)y * v.width + x;
+        float4 prev = *px;
+        for (uint32_t s = 0; s < v.spp; s++) {
+            float4 l = L[v.slot_base + s * npix + p];
+            float3 c;
+            if (v.flags & HH_FLAG_NO_CLAMP_BIT) c = make_float3(l.x, l.y, l.z);
+            else c = make_float3(fminf(l.x, 1.0f), fminf(l.y, 1.0f), fminf(l.z, 1.0f));
+            uint32_t f = v.first_frame + s;
+            if (f == 0) prev = make_float4(c.x, c.y, c.z, 1.0f);
+            else {
+                float ff = (float)f;
+                prev = make_float4(prev.x + (c.x - prev.x) / ff, prev.y + (c.y - prev.y) / ff, prev.z + (c.z - prev.z) / ff, 1.0f);
+            }
+        }
+        *px = prev;
+    }
+}
+
+// Megakernel variant: one thread runs a whole path (same device functions).
+template <int W, int BS>
+__global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
+                                                   uint32_t nv, uint32_t total, float4* __restrict__ Lout,
+                                                   unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t stack[kLdsStack * BS];
+    const uint32_t stride = gridDim.x * BS;
+    unsigned long long n_ext = 0, n_sh = 0;
+    for (uint32_t slot = blockIdx.x * BS + threadIdx.x; slot < total; slot += stride) {
+        SlotInfo si = slot_info(views, nv, slot);
+        const ViewGPU& V = views[si.view];
+        const SceneGPU& S = scenes[V.scene];
+        RNG rng = rng_init(si.x, si.y, V.first_frame + si.sample);
+        f3 o, d;
+        generate_ray(V, si.x, si.y, rng, o, d);
+        f3 T = mk(1.f, 1.f, 1.f), Lp = mk(0.f, 0.f, 0.f);
+        float tmin = 0.001f;
+        uint32_t rflags = 0;
+        for (uint32_t depth = 0;; depth++) {
+            HitRec h;
+            n_ext++;
+            trace_ray<W, false>(S, o, d, tmin, 10000.0f, rflags, h, &stack[threadIdx.x], BS);
+            if (!h.found) {
+                f3 e = sample_env(S, d);
+                Lp = Lp + (depth == 0 ? e : T * e);
+                break;
+            }
+            Surface p;
+            populate(S, h.inst, h.tri, h.u, h.v, p);
+            if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
+            ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, rng);
+            if (sr.want) {
+                HitRec sh;
+                n_sh++;
+                if (!trace_ray<W, true>(S, sr.o, sr.d, sr.tmin, sr.tmax, sr.flags, sh, &stack[threadIdx.x], BS))
+                    Lp = Lp + sr.contrib;
+            }
+            if ((V.flags & HH_FLAG_DIRECT_ONLY_BIT) != 0u || depth + 1 >= V.max_bounces) break;
+            f3 Wo = -d, Wi;
+            float pdf;
+            f3 brdf = sample_uber(p, Wo, rng, Wi, pdf);
+            float ct = clampf(dot(p.normal, Wi), 0.0f, 1.0f);
+            f3 Tn = (T * (brdf * ct)) / pdf;
+            float prob = fmaxf(Tn.x, fmaxf(Tn.y, Tn.z));
+            if (next_float(rng) > prob) break;
+            T = Tn * (1.0f / prob);
+            o = p.pos;
+            d = Wi;
+            tmin = 0.0001f;
+            rflags = HH_TRACE_OPAQUE_BIT;
+        }
+        Lout[slot] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        n_ext += __shfl_xor(n_ext, off);
+        n_sh += __shfl_xor(n_sh, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&stats[0], n_ext);
+        atomicAdd(&stats[1], n_sh);
+    }
+}
+
+// Debug / parity entry points ------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(256) k_trace_rays(const SceneGPU* __restrict__ scene, const float4* __restrict__ rays, uint32_t n,
+                                                    uint32_t flags, float4* __restrict__ out_hit, uint32_t* __restrict__ out_ids) {
+    __shared__ uint32_t stack[kLdsStack * 256];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const SceneGPU& S = *scene;
+    float4 o = rays[2 * i], d = rays[2 * i + 1];
+    HitRec h;
+    if (flags & HH_TRACE_TERMINATE_BIT)
+        trace_ray<W, true>(S, ld3(o), ld3(d), o.w, d.w, flags, h, &stack[threadIdx.x], 256);
+    else
+        trace_ray<W, false>(S, ld3(o), ld3(d), o.w, d.w, flags, h, &stack[threadIdx.x], 256);
+    out_hit[i] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.found ? h.inst : 0xFFFFFFFFu));
+    uint32_t geom = 0xFFFFFFFFu, prim = 0xFFFFFFFFu;
+    if (h.found) {
+        prim = __float_as_uint(S.tris[3 * (size_t)h.tri].w);
+        geom = __float_as_uint(S.tris[3 * (size_t)h.tri + 1].w) & 0x7FFFFFFFu;
+    }
+    out_ids[2 * i] = geom;
+    out_ids[2 * i + 1] = prim;
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) k_primary(const SceneGPU* __restrict__ scene, const ViewGPU* __restrict__ view, uint32_t w,
+                                                 uint32_t h_, uint32_t frame, float4* __restrict__ out_hit,
+                                                 uint32_t* __restrict__ out_ids) {
+    __shared__ uint32_t stack[kLdsStack * 256];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= w * h_) return;
+    uint32_t x = i % w, y = i / w;
+    const SceneGPU& S = *scene;
+    RNG rng = rng_init(x, y, frame);
+    f3 o, d;
+    generate_ray(*view, x, y, rng, o, d);
+    HitRec h;
+    trace_ray<W, false>(S, o, d, 0.001f, 10000.0f, 0u, h, &stack[threadIdx.x], 256);
+    out_hit[i] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.found ? h.inst : 0xFFFFFFFFu));
+    uint32_t geom = 0xFFFFFFFFu, prim = 0xFFFFFFFFu;
+    if (h.found) {
+        prim = __float_as_uint(S.tris[3 * (size_t)h.tri].w);
+        geom = __float_as_uint(S.tris[3 * (size_t)h.tri + 1].w) & 0x7FFFFFFFu;
+    }
+    out_ids[2 * i] = geom;
+    out_ids[2 * i + 1] = prim;
+}
+
+// tone_map.frag:20-51
+__global__ void k_tonemap(const float4* __restrict__ src, uint32_t n, float exposure, uint32_t op, uchar4* __restrict__ dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c = src[i];
+    float v[3] = {c.x * exposure, c.y * exposure, c.z * exposure};
+    unsigned char o[3];
+    for (int k = 0; k < 3; k++) {
+        float x = v[k];
+        if (op == 0u) x = fminf(fmaxf((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f), 0.0f), 1.0f);
+        else if (op == 1u) x = x / (1.0f + x);
+        x = powf(x, 1.0f / 2.2f);
+        x = fminf(fmaxf(x, 0.0f), 1.0f);
+        o[k] = (unsigned char)floorf(x * 255.0f + 0.5f);
+    }
+    dst[i] = make_uchar4(o[0], o[1], o[2], 255);
+}
+
+// procedural_sky.frag:48-75 evaluated at every cube texel centre.
+__global__ void k_sky_bake(float4* __restrict__ faces, uint32_t n, const float* __restrict__ coef, float sx, float sy, float sz) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t total = 6u * n * n;
+    if (i >= total) return;
+    uint32_t face = i / (n * n), rem = i % (n * n), j = rem / n, k = rem % n;
+    float sc = 2.0f * (((float)k + 0.5f) / (float)n) - 1.0f;
+    float tc = 2.0f * (((float)j + 0.5f) / (float)n) - 1.0f;
+    f3 d;
+    switch (face) {
+        case 0: d = mk(1.0f, -tc, -sc); break;
+        case 1: d = mk(-1.0f, -tc, sc); break;
+        case 2: d = mk(sc, 1.0f, tc); break;
+        case 3: d = mk(sc, -1.0f, -tc); break;
+        case 4: d = mk(sc, -tc, 1.0f); break;
+        default: d = mk(-sc, -tc, -1.0f); break;
+    }
+    d = normalize(d);
+    float cos_theta = clampf(d.y, 0.0f, 1.0f);
+    float cos_gamma = clampf(d.x * sx + d.y * sy + d.z * sz, 0.0f, 1.0f);
+    float gamma = acosf(cos_gamma);
+    float out[3];
+    for (int c = 0; c < 3; c++) {
+        float A = coef[0 + c], B = coef[3 + c], C = coef[6 + c], D = coef[9 + c], E = coef[12 + c], F = coef[15 + c],
+              G = coef[18 + c], H = coef[21 + c], I = coef[24 + c], Z = coef[27 + c];
+        float chi = (1.0f + cos_gamma * cos_gamma) / powf(1.0f + H * H - 2.0f * cos_gamma * H, 1.5f);
+        float hw = (1.0f + A * expf(B / (cos_theta + 0.01f))) *
+                   (C + D * expf(E * gamma) + F * (cos_gamma * cos_gamma) + G * chi + I * sqrtf(cos_theta));
+        out[c] = Z * hw;
+    }
+    faces[i] = make_float4(out[0], out[1], out[2], 1.0f);
+}
+
+__global__ void k_fill_f4(float4* p, float4 v, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+template <int W, int BS>
+static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, const ViewGPU* d_views, const ViewGPU* h_views,
+                             uint32_t nv, uint32_t total, uint32_t max_bounces, Wavefront& wf, bool mega) {
+    hipStream_t st = cfg.stream;
+    const uint32_t grid_cap = (uint32_t)cfg.num_cu * (uint32_t)cfg.tune.blocks_per_cu;
+    const uint32_t grid = std::max(1u, std::min(grid_cap, cdiv(total, BS)));
+    if (mega) {
+        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats);
+    } else {
+        k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, wf.counters);
+        for (uint32_t d = 0; d < max_bounces; d++) {
+            const int a = d & 1, b = a ^ 1;
+            k_extend<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a], wf.counters + a,
+                                                 d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit, wf.hit_tri, wf.stats);
+            k_reset2<<<1, 64, 0, st>>>(wf.counters + b, wf.counters + 2);
+            k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], wf.counters + a, wf.hit,
+                                             wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b], wf.counters + b,
+                                             wf.s_o, wf.s_d, wf.s_c, wf.s_slot, wf.counters + 2);
+            k_shadow<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, wf.counters + 2, wf.L,
+                                                 wf.stats);
+        }
+    }
+    uint32_t maxpix = 0;
+    for (uint32_t i = 0; i < nv; i++) maxpix = std::max(maxpix, h_views[i].tile_w * h_views[i].tile_h);
+    dim3 ag(std::max(1u, std::min(cdiv(maxpix, 256), 1024u)), nv);
+    k_accumulate<<<ag, 256, 0, st>>>(d_views, nv, wf.L);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const SceneGPU* h_scenes, const ViewGPU* d_views,
+                        const ViewGPU* h_views, uint32_t nv, uint32_t total, uint32_t max_bounces, Wavefront& wf,
+                        bool mega) {
+    const int W = h_scenes[0].bvh_width;
+    const int bs = cfg.tune.extend_block;
+#define HH_DISPATCH(WW, BB) return frame_impl<WW, BB>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
+    if (W == 8) {
+        if (bs == 64) HH_DISPATCH(8, 64);
+        if (bs == 128) HH_DISPATCH(8, 128);
+        HH_DISPATCH(8, 256);
+    }
+    if (bs == 64) HH_DISPATCH(4, 64);
+    if (bs == 128) HH_DISPATCH(4, 128);
+    HH_DISPATCH(4, 256);
+#undef HH_DISPATCH
+}
+
+hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,
+                             uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_ids) {
+    if (n == 0) return hipSuccess;
+    if (h_scene.bvh_width == 8) k_trace_rays<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids);
+    else k_trace_rays<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
+                               uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_ids) {
+    uint32_t n = w * h;
+    if (h_scene.bvh_width == 8) k_primary<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids);
+    else k_primary<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst) {
+    k_tonemap<<<cdiv(n, 256), 256, 0, st>>>(src, n, exposure, op, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const float* d_coef, const float sun[3]) {
+    uint32_t total = 6u * size * size;
+    k_sky_bake<<<cdiv(total, 256), 256, 0, st>>>(faces, size, d_coef, sun[0], sun[1], sun[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f4(hipStream_t st, float4* p, float4 v, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    k_fill_f4<<<cdiv(n, 256), 256, 0, st>>>(p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace hh

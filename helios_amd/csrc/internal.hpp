@@ -1,0 +1,93 @@
+// internal.hpp — host-side declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gpu_scene.hpp"
+
+#define HH_TRY(expr)                             \
+    do {                                         \
+        hipError_t _e = (expr);                  \
+        if (_e != hipSuccess) return _e;         \
+    } while (0)
+
+namespace hh {
+
+struct BuildScratch {
+    uint32_t cap = 0;
+    uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *order = nullptr;
+    int2 *childs = nullptr, *range = nullptr, *jobs_a = nullptr, *jobs_b = nullptr;
+    uint32_t *parent = nullptr, *flags = nullptr, *counters = nullptr, *cbounds = nullptr;
+    float4 *blo = nullptr, *bhi = nullptr, *aabb_lo = nullptr, *aabb_hi = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    hipError_t ensure(uint32_t n);
+    void release();
+    uint32_t* vals_out_tmp(uint32_t) { return order; }
+    ~BuildScratch() { release(); }
+};
+
+hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
+                      uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
+                      float root_box[6]);
+hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes);
+
+// Wavefront state of one render call (device buffers sized for max_paths).
+struct Wavefront {
+    uint32_t capacity = 0;
+    // path state by slot
+    float4* T = nullptr;
+    float4* L = nullptr;
+    uint2* rng = nullptr;
+    // extension ray queues (ping-pong)
+    float4* q_o[2] = {nullptr, nullptr};
+    float4* q_d[2] = {nullptr, nullptr};
+    uint32_t* q_slot[2] = {nullptr, nullptr};
+    // hits of the current extension queue
+    float4* hit = nullptr;       // t, u, v, bits(instance)
+    uint32_t* hit_tri = nullptr; // global triangle index, 0xFFFFFFFF on miss
+    // shadow queue
+    float4* s_o = nullptr;
+    float4* s_d = nullptr;
+    float4* s_c = nullptr;       // contribution rgb, bits(ray flags)
+    uint32_t* s_slot = nullptr;
+    // counters: [0,1] extension queue sizes, [2] shadow size, [3..] stats
+    uint32_t* counters = nullptr;
+    unsigned long long* stats = nullptr;  // [0] extension rays, [1] shadow rays
+    hipError_t ensure(uint32_t n);
+    void release();
+    ~Wavefront() { release(); }
+};
+
+struct Tuning {
+    int bvh_width = 4;
+    int extend_block = 256;
+    int shade_block = 256;
+    uint32_t queue_capacity = 0;
+    int blocks_per_cu = 8;
+    int leaf_size = 4;
+};
+
+// Kernel launchers (integrator.hip)
+struct LaunchCfg {
+    hipStream_t stream;
+    int num_cu;
+    Tuning tune;
+};
+hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const SceneGPU* h_scenes, const ViewGPU* d_views,
+                        const ViewGPU* h_views, uint32_t num_views, uint32_t total_paths, uint32_t max_bounces,
+                        Wavefront& wf, bool megakernel);
+hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,
+                             uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_hit_ids);
+hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
+                               uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_hit_ids);
+hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst);
+hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const float* d_coef, const float sun[3]);
+
+// Host Hosek-Wilkie fit (sky_host.cpp)
+void sky_coefficients(float turbidity, float albedo, float normalized_sun_y, const float sun_dir[3], float out[30]);
+hipError_t launch_fill_f4(hipStream_t st, float4* p, float4 v, uint32_t n);
+
+}  // namespace hh

@@ -1,0 +1,268 @@
+// traverse.hpp — two-level wide-BVH traversal with watertight triangle tests.
+//
+// Replaces the closest-hit / any-hit / terminate-on-first-hit semantics of
+// traceRayEXT over the instance TLAS and per-mesh BLAS
+// (src/engine/shader/path_trace_rgen.glsl:205-215,
+//  src/engine/shader/path_trace_rchit.glsl:438-448,523-533; any-hit
+//  alpha test path_trace_rahit.glsl:174-188; BLAS geometry opaque flag
+//  src/engine/resource/mesh.cpp:75-76; instance cull disabled
+//  src/engine/resource/scene.cpp:1257).
+//
+// Per lane: a short stack of kLdsStack entries in LDS (interleaved by lane so
+// the 64 lanes of a wave hit 64 different banks) that spills into a private
+// array when deeper. Boxes use the robust slab test of Ize ("Robust BVH Ray
+// Traversal", JCGT 2013); triangles use the watertight test of Woop, Benthin
+// and Wald (JCGT 2013) with its double-precision fallback on zero edge
+// functions. Equal-t hits are resolved to the smallest (instance, geometry,
+// primitive), so the result does not depend on traversal order.
+#pragma once
+
+#include "gpu_common.hpp"
+#include "gpu_scene.hpp"
+
+namespace hh {
+
+constexpr int kLdsStack = 16;
+constexpr int kSpillStack = 112;
+
+struct HitRec {
+    float t, u, v;
+    uint32_t inst;
+    uint32_t tri;  // global triangle index
+    bool found;
+};
+
+struct WT {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+__device__ inline float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ inline WT wt_pre(f3 d) {
+    WT p;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    p.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    p.kx = p.kz + 1;
+    if (p.kx == 3) p.kx = 0;
+    p.ky = p.kx + 1;
+    if (p.ky == 3) p.ky = 0;
+    if (comp(d, p.kz) < 0.0f) {
+        int t = p.kx;
+        p.kx = p.ky;
+        p.ky = t;
+    }
+    float dz = comp(d, p.kz);
+    p.Sx = comp(d, p.kx) / dz;
+    p.Sy = comp(d, p.ky) / dz;
+    p.Sz = 1.0f / dz;
+    return p;
+}
+
+// Woop-Benthin-Wald watertight test; the operation order is fixed (the
+// library is compiled without FP contraction) and equals the CPU oracle's.
+__device__ inline bool tri_hit(const WT& pr, f3 o, f3 v0, f3 v1, f3 v2, float tmin, float tmax, float& t, float& u,
+                               float& v) {
+    f3 A = v0 - o, B = v1 - o, C = v2 - o;
+    float Akz = comp(A, pr.kz), Bkz = comp(B, pr.kz), Ckz = comp(C, pr.kz);
+    float Ax = comp(A, pr.kx) - pr.Sx * Akz, Ay = comp(A, pr.ky) - pr.Sy * Akz;
+    float Bx = comp(B, pr.kx) - pr.Sx * Bkz, By = comp(B, pr.ky) - pr.Sy * Bkz;
+    float Cx = comp(C, pr.kx) - pr.Sx * Ckz, Cy = comp(C, pr.ky) - pr.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = pr.Sz * Akz, Bz = pr.Sz * Bkz, Cz = pr.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+// World -> object (row-major 3x4), fixed operation order.
+__device__ inline void xform_ray(const float* m, f3 o, f3 d, f3& oo, f3& od) {
+    oo.x = ((m[0] * o.x + m[1] * o.y) + m[2] * o.z) + m[3];
+    oo.y = ((m[4] * o.x + m[5] * o.y) + m[6] * o.z) + m[7];
+    oo.z = ((m[8] * o.x + m[9] * o.y) + m[10] * o.z) + m[11];
+    od.x = (m[0] * d.x + m[1] * d.y) + m[2] * d.z;
+    od.y = (m[4] * d.x + m[5] * d.y) + m[6] * d.z;
+    od.z = (m[8] * d.x + m[9] * d.y) + m[10] * d.z;
+}
+
+// ---------------------------------------------------------------------------
+// Texture sampling (Vulkan bilinear, LOD 0, REPEAT) and the any-hit alpha test.
+// ---------------------------------------------------------------------------
+__device__ inline float lerpf(float a, float b, float w) { return a + (b - a) * w; }
+__device__ inline int wrapi(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+__device__ inline float4 sample_tex(const SceneGPU& S, int slot, float s, float t) {
+    TexDesc td = S.tex[slot];
+    float x = s * (float)td.w - 0.5f, y = t * (float)td.h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float ax = x - fx, ay = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    int i1 = wrapi(i0 + 1, (int)td.w), j1 = wrapi(j0 + 1, (int)td.h);
+    i0 = wrapi(i0, (int)td.w);
+    j0 = wrapi(j0, (int)td.h);
+    const float4* base = S.texels + td.offset;
+    float4 p00 = base[j0 * td.w + i0], p10 = base[j0 * td.w + i1];
+    float4 p01 = base[j1 * td.w + i0], p11 = base[j1 * td.w + i1];
+    float4 r;
+    r.x = lerpf(lerpf(p00.x, p10.x, ax), lerpf(p01.x, p11.x, ax), ay);
+    r.y = lerpf(lerpf(p00.y, p10.y, ax), lerpf(p01.y, p11.y, ax), ay);
+    r.z = lerpf(lerpf(p00.z, p10.z, ax), lerpf(p01.z, p11.z, ax), ay);
+    r.w = lerpf(lerpf(p00.w, p10.w, ax), lerpf(p01.w, p11.w, ax), ay);
+    return r;
+}
+
+// path_trace_rahit.glsl:162-188: alpha of the candidate hit.
+__device__ inline float hit_alpha(const SceneGPU& S, uint32_t inst, uint32_t geom, uint32_t prim, float u, float v) {
+    uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
+    const MaterialData& m = S.mats[info.y];
+    if (m.tex0[0] == -1) return m.albedo[3];
+    uint32_t mesh = S.inst_trav[inst].mesh;
+    uint32_t p = prim + info.x;
+    const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * p;
+    const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
+    float4 t0 = vb[5 * (size_t)ix[0] + 1], t1 = vb[5 * (size_t)ix[1] + 1], t2 = vb[5 * (size_t)ix[2] + 1];
+    float bx = (1.0f - u) - v, by = u, bz = v;
+    float tu = (t0.x * bx + t1.x * by) + t2.x * bz;
+    float tv = (t0.y * bx + t1.y * by) + t2.y * bz;
+    return sample_tex(S, m.tex0[0], tu, tv).w;
+}
+
+__device__ inline bool key_less(uint32_t i1, uint32_t g1, uint32_t p1, uint32_t i2, uint32_t g2, uint32_t p2) {
+    if (i1 != i2) return i1 < i2;
+    if (g1 != g2) return g1 < g2;
+    return p1 < p2;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal. ANY = stop at the first accepted hit (visibility rays).
+// rflags: HH_TRACE_OPAQUE (bit 0) disables the any-hit alpha test.
+// `lds` points at this lane's first stack word; consecutive entries are
+// `stride` words apart.
+// ---------------------------------------------------------------------------
+template <int W, bool ANY>
+__device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, float tmax, uint32_t rflags, HitRec& h,
+                                 uint32_t* lds, int stride) {
+    const WideNode<W>* nodes = (const WideNode<W>*)S.nodes;
+    const float4* tris = S.tris;
+    f3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+    f3 o = wo, inv = winv;
+    WT wt = wt_pre(wd);
+    int inst = -1;
+    uint32_t sp = 0;
+    uint32_t spill[kSpillStack];
+    h.found = false;
+    h.t = tmax;
+    h.u = h.v = 0.0f;
+    h.inst = 0xFFFFFFFFu;
+    h.tri = 0xFFFFFFFFu;
+    uint32_t hgeom = 0xFFFFFFFFu, hprim = 0xFFFFFFFFu;
+    const bool alpha_on = (rflags & 1u) == 0u;
+    // Entries beyond the stack capacity are dropped (never written out of
+    // bounds); the capacity is far above the depth of the built trees.
+    auto push = [&](uint32_t r) {
+        if (sp < (uint32_t)kLdsStack) lds[sp * stride] = r;
+        else if (sp < (uint32_t)(kLdsStack + kSpillStack)) spill[sp - kLdsStack] = r;
+        else return;
+        sp++;
+    };
+    uint32_t cur = S.tlas_root;
+    // Hard bound on loop trips: every wave leaves the loop even if a
+    // malformed tree contained a cycle.
+    for (uint32_t trips = 0; trips < (1u << 22); trips++) {
+        if ((cur & kLeafBit) == 0u) {
+            const WideNode<W>& nd = nodes[cur];
+            float tn[W];
+            uint32_t ref[W];
+            const uint32_t nc = nd.nchild;
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                float t0x = (nd.lox[k] - o.x) * inv.x, t1x = (nd.hix[k] - o.x) * inv.x;
+                float t0y = (nd.loy[k] - o.y) * inv.y, t1y = (nd.hiy[k] - o.y) * inv.y;
+                float t0z = (nd.loz[k] - o.z) * inv.z, t1z = (nd.hiz[k] - o.z) * inv.z;
+                float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+                float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), h.t));
+                bool hit = (uint32_t)k < nc && a <= b * 1.0000004f;
+                tn[k] = hit ? a : INFINITY;
+                ref[k] = nd.child[k];
+            }
+            // sort ascending by entry distance (small network, registers only)
+#pragma unroll
+            for (int i = 1; i < W; i++)
+#pragma unroll
+                for (int j = W - 1; j >= i; j--)
+                    if (tn[j] < tn[j - 1]) {
+                        float tt = tn[j]; tn[j] = tn[j - 1]; tn[j - 1] = tt;
+                        uint32_t rr = ref[j]; ref[j] = ref[j - 1]; ref[j - 1] = rr;
+                    }
+            if (tn[0] == INFINITY) goto pop;
+#pragma unroll
+            for (int k = W - 1; k >= 1; k--)
+                if (tn[k] != INFINITY) push(ref[k]);
+            cur = ref[0];
+            continue;
+        } else if (cur == kMarker) {
+            o = wo;
+            inv = winv;
+            inst = -1;
+            goto pop;
+        } else if (inst < 0) {
+            inst = (int)leaf_start(cur);
+            const InstTrav& it = S.inst_trav[inst];
+            f3 od;
+            xform_ray(it.w2o, wo, wd, o, od);
+            inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+            wt = wt_pre(od);
+            push(kMarker);
+            cur = it.blas_root;
+            continue;
+        } else {
+            const uint32_t first = leaf_start(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t ti = first + k;
+                float4 a = tris[3 * ti], b = tris[3 * ti + 1], c = tris[3 * ti + 2];
+                float t, u, v;
+                if (!tri_hit(wt, o, ld3(a), ld3(b), ld3(c), tmin, h.t, t, u, v)) continue;
+                const uint32_t prim = __float_as_uint(a.w);
+                const uint32_t gw = __float_as_uint(b.w);
+                const uint32_t geom = gw & 0x7FFFFFFFu;
+                if (h.found && t == h.t && !key_less((uint32_t)inst, geom, prim, h.inst, hgeom, hprim)) continue;
+                if (alpha_on && (gw & 0x80000000u) == 0u) {
+                    if (hit_alpha(S, (uint32_t)inst, geom, prim, u, v) < 0.1f) continue;
+                }
+                h.found = true;
+                h.t = t;
+                h.u = u;
+                h.v = v;
+                h.inst = (uint32_t)inst;
+                h.tri = ti;
+                hgeom = geom;
+                hprim = prim;
+                if (ANY) return true;
+            }
+            goto pop;
+        }
+    pop:
+        if (sp == 0) break;
+        sp--;
+        cur = sp < (uint32_t)kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+    }
+    return h.found;
+}
+
+}  // namespace hh

@@ -1,0 +1,380 @@
+"""Python mirror of the reference's render-path interface over the C ABI.
+
+Class and method names follow the reference:
+  PathIntegrator  include/gfx/path_integrator.h:9-62 (render, restart_bake,
+                  set_max_ray_bounces, set_max_samples, set_shadow_ray_bias,
+                  set_tiled, num_accumulated_samples, num_target_samples, ...)
+  Scene           include/resource/scene.h:386-470 (built from a SceneDesc)
+  Renderer        tone map / exposure / operator (include/gfx/renderer.h:75-97)
+The GPU library (libhelios_hip.so, HIP for gfx950) does all the work; this
+module only marshals arguments. It raises HeliosError on any non-OK status and
+refuses to run when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+from .abi import ptr
+
+LIB_PATH = Path(__file__).resolve().parent / "libhelios_hip.so"
+
+_lib = None
+
+
+class HeliosError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{abi.STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+# name -> argtypes (restype is int status unless noted)
+_SIGS = {
+    "hh_context_create": [C.c_int, C.POINTER(C.c_void_p)],
+    "hh_context_destroy": [C.c_void_p],
+    "hh_sync": [C.c_void_p],
+    "hh_context_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "hh_context_set_tuning": [C.c_void_p, C.POINTER(abi.Tuning)],
+    "hh_context_get_tuning": [C.c_void_p, C.POINTER(abi.Tuning)],
+    "hh_scene_create": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "hh_scene_destroy": [C.c_void_p],
+    "hh_scene_add_texture": [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)],
+    "hh_scene_add_material": [C.c_void_p, C.POINTER(abi.MaterialDesc), C.POINTER(C.c_uint32)],
+    "hh_scene_add_mesh": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                          C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)],
+    "hh_scene_add_instance": [C.c_void_p, C.POINTER(abi.InstanceDesc), C.POINTER(C.c_uint32)],
+    "hh_scene_set_instance_transform": [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p],
+    "hh_scene_add_light": [C.c_void_p, C.POINTER(abi.LightDesc), C.POINTER(C.c_uint32)],
+    "hh_scene_set_ibl": [C.c_void_p, C.c_uint32, C.c_void_p],
+    "hh_scene_set_sky": [C.c_void_p, C.POINTER(abi.SkyParams)],
+    "hh_scene_commit": [C.c_void_p],
+    "hh_scene_num_lights": [C.c_void_p, C.POINTER(C.c_uint32)],
+    "hh_scene_read_lights": [C.c_void_p, C.c_void_p, C.c_uint32],
+    "hh_scene_read_env": [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)],
+    "hh_scene_sky_coefficients": [C.c_void_p, C.c_void_p],
+    "hh_scene_bvh_stats": [C.c_void_p, C.c_void_p],
+    "hh_surface_create": [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)],
+    "hh_surface_destroy": [C.c_void_p],
+    "hh_surface_clear": [C.c_void_p],
+    "hh_surface_read": [C.c_void_p, C.c_void_p],
+    "hh_surface_write": [C.c_void_p, C.c_void_p],
+    "hh_surface_device_ptr": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "hh_tonemap": [C.c_void_p, C.c_float, C.c_uint32, C.c_void_p],
+    "hh_integrator_create": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "hh_integrator_destroy": [C.c_void_p],
+    "hh_integrator_set_settings": [C.c_void_p, C.POINTER(abi.Settings)],
+    "hh_integrator_get_settings": [C.c_void_p, C.POINTER(abi.Settings)],
+    "hh_integrator_restart_bake": [C.c_void_p],
+    "hh_integrator_render": [C.c_void_p, C.c_void_p, C.POINTER(abi.Camera), C.c_void_p],
+    "hh_integrator_render_frames": [C.c_void_p, C.c_void_p, C.POINTER(abi.Camera), C.c_void_p, C.c_uint32],
+    "hh_integrator_counters": [C.c_void_p, C.c_void_p],
+    "hh_integrator_ray_stats": [C.c_void_p, C.c_void_p],
+    "hh_compute_launch_consts": [C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.POINTER(abi.Settings), C.POINTER(abi.LaunchConsts)],
+    "hh_render_batch": [C.c_void_p, C.c_void_p, C.c_uint32],
+    "hh_trace_rays": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
+    "hh_primary_hits": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p],
+    "hh_sky_fit": [C.POINTER(abi.SkyParams), C.c_void_p, C.c_void_p],
+}
+
+
+def lib():
+    """Load libhelios_hip.so. Raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HeliosError(4, f"{LIB_PATH} is missing; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.hh_last_error.restype = C.c_char_p
+        L.hh_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        raise HeliosError(status, lib().hh_last_error().decode(errors="replace"))
+
+
+def _struct(cls, d):
+    s = cls()
+    for k, v in d.items():
+        f = getattr(s, k)
+        if isinstance(f, C.Array):
+            for i, x in enumerate(v):
+                f[i] = x
+        else:
+            setattr(s, k, v)
+    return s
+
+
+def camera_struct(cam):
+    return cam if isinstance(cam, abi.Camera) else _struct(abi.Camera, cam)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().hh_context_create(device, C.byref(self.h)))
+
+    def sync(self):
+        check(lib().hh_sync(self.h))
+
+    def set_tuning(self, bvh_width=4, extend_block=256, queue_capacity=0, persistent_blocks_per_cu=8):
+        t = abi.Tuning(bvh_width, extend_block, extend_block, queue_capacity, persistent_blocks_per_cu)
+        check(lib().hh_context_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = abi.Tuning()
+        check(lib().hh_context_get_tuning(self.h, C.byref(t)))
+        return dict(bvh_width=t.bvh_width, extend_block=t.extend_block, queue_capacity=t.queue_capacity,
+                    persistent_blocks_per_cu=t.persistent_blocks_per_cu)
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().hh_context_stream(self.h, C.byref(s)))
+        return s.value
+
+    def close(self):
+        if self.h:
+            lib().hh_context_destroy(self.h)
+            self.h = None
+
+
+class Scene:
+    """helios::Scene built from a scenegen.SceneDesc and committed to the GPU."""
+
+    def __init__(self, ctx, desc=None, commit=True):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().hh_scene_create(ctx.h, C.byref(self.h)))
+        self.desc = desc
+        if desc is not None:
+            self.load(desc)
+            if commit:
+                self.commit()
+
+    def load(self, d):
+        L = lib()
+        out = C.c_uint32()
+        for (w, h, fmt, data) in d.textures:
+            data = np.ascontiguousarray(data)
+            check(L.hh_scene_add_texture(self.h, w, h, fmt, ptr(data), C.byref(out)))
+        for m in d.materials:
+            check(L.hh_scene_add_material(self.h, C.byref(_struct(abi.MaterialDesc, m)), C.byref(out)))
+        for me in d.meshes:
+            sm = np.ascontiguousarray(np.array(me.submeshes, np.uint32).reshape(-1, 5))
+            mats = np.ascontiguousarray(np.array(me.materials, np.uint32))
+            v = np.ascontiguousarray(me.vertices)
+            i = np.ascontiguousarray(me.indices, dtype=np.uint32)
+            check(L.hh_scene_add_mesh(self.h, ptr(v), len(v), ptr(i), len(i), ptr(sm), len(sm), ptr(mats), len(mats),
+                                      C.byref(out)))
+        for (model, normal, mesh, ov) in d.instances:
+            check(L.hh_scene_add_instance(self.h, C.byref(_struct(abi.InstanceDesc, dict(
+                model=list(model), normal=list(normal), mesh=mesh, material_override=ov))), C.byref(out)))
+        for l in d.lights:
+            check(L.hh_scene_add_light(self.h, C.byref(_struct(abi.LightDesc, l)), C.byref(out)))
+        if d.ibl is not None:
+            size, faces = d.ibl
+            faces = np.ascontiguousarray(faces, np.float32)
+            check(L.hh_scene_set_ibl(self.h, size, ptr(faces)))
+        check(L.hh_scene_set_sky(self.h, C.byref(abi.SkyParams(*d.sky))))
+
+    def commit(self):
+        check(lib().hh_scene_commit(self.h))
+
+    def num_lights(self):
+        n = C.c_uint32()
+        check(lib().hh_scene_num_lights(self.h, C.byref(n)))
+        return n.value
+
+    def lights(self):
+        n = self.num_lights()
+        out = np.zeros((max(n, 1), 16), np.float32)
+        check(lib().hh_scene_read_lights(self.h, ptr(out), n))
+        return out[:n]
+
+    def env(self):
+        n = C.c_uint32()
+        check(lib().hh_scene_read_env(self.h, None, C.byref(n)))
+        out = np.zeros((6, n.value, n.value, 4), np.float32)
+        check(lib().hh_scene_read_env(self.h, ptr(out), C.byref(n)))
+        return out
+
+    def sky_coefficients(self):
+        out = np.zeros(30, np.float32)
+        check(lib().hh_scene_sky_coefficients(self.h, ptr(out)))
+        return out.reshape(10, 3)
+
+    def bvh_stats(self):
+        out = np.zeros(4, np.uint64)
+        check(lib().hh_scene_bvh_stats(self.h, ptr(out)))
+        return dict(blas_nodes=int(out[0]), triangles=int(out[1]), tlas_nodes=int(out[2]), meshes=int(out[3]))
+
+    def trace_rays(self, rays, flags=0):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        out = np.zeros(len(rays), abi.HIT_DTYPE)
+        check(lib().hh_trace_rays(self.h, ptr(rays), len(rays), flags, ptr(out)))
+        return out
+
+    def primary_hits(self, camera, width, height, frame=0):
+        out = np.zeros((height, width), abi.HIT_DTYPE)
+        check(lib().hh_primary_hits(self.h, C.byref(camera_struct(camera)), width, height, frame, ptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().hh_scene_destroy(self.h)
+            self.h = None
+
+
+class Surface:
+    """The RGBA32F output image (row 0 = NDC y = -1)."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.h = C.c_void_p()
+        check(lib().hh_surface_create(ctx.h, width, height, C.byref(self.h)))
+
+    def clear(self):
+        check(lib().hh_surface_clear(self.h))
+
+    def read(self):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        check(lib().hh_surface_read(self.h, ptr(out)))
+        return out
+
+    def write(self, img):
+        img = np.ascontiguousarray(img, np.float32)
+        check(lib().hh_surface_write(self.h, ptr(img)))
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        check(lib().hh_surface_device_ptr(self.h, C.byref(p)))
+        return p.value
+
+    def tone_map(self, exposure=1.0, operator=0):
+        """tone_map.frag: ACES (0) / Reinhard (1), exposure, gamma 1/2.2 -> RGBA8."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        check(lib().hh_tonemap(self.h, exposure, operator, ptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().hh_surface_destroy(self.h)
+            self.h = None
+
+
+class PathIntegrator:
+    """helios::PathIntegrator over the GPU wavefront integrator."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().hh_integrator_create(ctx.h, C.byref(self.h)))
+
+    def _settings(self):
+        s = abi.Settings()
+        check(lib().hh_integrator_get_settings(self.h, C.byref(s)))
+        return s
+
+    def _set(self, **kw):
+        s = self._settings()
+        for k, v in kw.items():
+            setattr(s, k, v)
+        check(lib().hh_integrator_set_settings(self.h, C.byref(s)))
+
+    # path_integrator.h:18-37
+    def max_ray_bounces(self):
+        return self._settings().max_ray_bounces
+
+    def max_samples(self):
+        return self._settings().max_samples
+
+    def shadow_ray_bias(self):
+        return self._settings().shadow_ray_bias
+
+    def is_tiled(self):
+        return bool(self._settings().tiled)
+
+    def set_max_ray_bounces(self, n):
+        self._set(max_ray_bounces=n)
+
+    def set_max_samples(self, n):
+        self._set(max_samples=n)
+
+    def set_shadow_ray_bias(self, b):
+        self._set(shadow_ray_bias=b)
+
+    def set_tiled(self, tiled):
+        self._set(tiled=int(bool(tiled)))
+
+    def set_flags(self, flags):
+        self._set(flags=flags)
+
+    def restart_bake(self):
+        check(lib().hh_integrator_restart_bake(self.h))
+
+    def _counters(self):
+        out = np.zeros(4, np.uint32)
+        check(lib().hh_integrator_counters(self.h, ptr(out)))
+        return out
+
+    def num_accumulated_samples(self):
+        return int(self._counters()[0])
+
+    def num_target_samples(self):
+        return int(self._counters()[1])
+
+    def tile_idx(self):
+        return int(self._counters()[2])
+
+    def render(self, scene, camera, surface, frames=1):
+        check(lib().hh_integrator_render_frames(self.h, scene.h, C.byref(camera_struct(camera)), surface.h, frames))
+
+    def ray_stats(self):
+        out = np.zeros(3, np.uint64)
+        check(lib().hh_integrator_ray_stats(self.h, ptr(out)))
+        return dict(extension_rays=int(out[0]), shadow_rays=int(out[1]), frames=int(out[2]))
+
+    def close(self):
+        if self.h:
+            lib().hh_integrator_destroy(self.h)
+            self.h = None
+
+
+def render_batch(ctx, views):
+    """views: list of dict(scene, surface, camera, spp, max_ray_bounces, shadow_ray_bias, first_frame)."""
+    arr = (abi.View * len(views))()
+    for i, v in enumerate(views):
+        arr[i].scene = v["scene"].h.value
+        arr[i].surface = v["surface"].h.value
+        arr[i].camera = camera_struct(v["camera"])
+        arr[i].spp = v.get("spp", 1)
+        arr[i].max_ray_bounces = v.get("max_ray_bounces", 8)
+        arr[i].shadow_ray_bias = v.get("shadow_ray_bias", 0.0)
+        arr[i].first_frame = v.get("first_frame", 0)
+    check(lib().hh_render_batch(ctx.h, arr, len(views)))
+
+
+def launch_consts(camera, width, height, num_lights=1, frame=0, max_ray_bounces=5):
+    c = abi.LaunchConsts()
+    st = abi.Settings(max_ray_bounces, 5000, 0.0, 0, 0, 0)
+    check(lib().hh_compute_launch_consts(C.byref(camera_struct(camera)), width, height, num_lights, frame,
+                                         C.byref(st), C.byref(c)))
+    return c
+
+
+def sky_fit(turbidity, albedo, normalized_sun_y, sun_dir):
+    out = np.zeros(30, np.float32)
+    sd = np.asarray(sun_dir, np.float32)
+    check(lib().hh_sky_fit(C.byref(abi.SkyParams(turbidity, albedo, normalized_sun_y, 512)), ptr(sd), ptr(out)))
+    return out.reshape(10, 3)

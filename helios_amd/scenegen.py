@@ -1,0 +1,426 @@
+"""Deterministic synthetic scenes for the path-tracing workloads.
+
+A scene is described by ``SceneDesc``: textures, materials, meshes (80-B
+Vertex arrays + uint32 indices + submeshes, the reference's Mesh::create
+inputs, src/engine/resource/mesh.cpp:27-42), mesh-node instances, light nodes,
+an optional IBL cube or Hosek-Wilkie sky parameters, a camera and the render
+settings. The same description feeds the GPU library and the CPU oracle.
+
+Workloads
+  * ``cornell_box``      — 256x256 Cornell box: 5 wall quads (10 tris), two
+                           axis-aligned boxes (24 tris), a 0.5x0.5 area light
+                           with emission (17, 12, 4); 40 degree pinhole camera.
+  * ``icosphere_field``  — seeded icospheres (level 5 = 20,480 tris each; 64 of
+                           them = 1,310,720 tris) displaced radially by 3-octave
+                           value noise (5 %), on a 200x200 ground plane, GGX
+                           metallic-roughness materials, 2,000 alpha-tested leaf
+                           quads, Hosek-Wilkie sky (turbidity 3, sun at 30
+                           degrees) and 4 point lights; 1920x1080, 8 bounces.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .abi import VERTEX_DTYPE, LIGHT_DIRECTIONAL, LIGHT_POINT, LIGHT_SPOT, TEXTURE_RGBA8_SRGB
+
+
+@dataclass
+class MeshDesc:
+    vertices: np.ndarray          # VERTEX_DTYPE
+    indices: np.ndarray           # uint32
+    submeshes: list               # [(mat_idx, index_count, vertex_count, base_vertex, base_index)]
+    materials: list               # material ids, indexed by submesh mat_idx
+
+
+@dataclass
+class SceneDesc:
+    textures: list = field(default_factory=list)    # [(w, h, fmt, uint8/float32 array)]
+    materials: list = field(default_factory=list)   # [dict(MaterialDesc fields)]
+    meshes: list = field(default_factory=list)      # [MeshDesc]
+    instances: list = field(default_factory=list)   # [(model 4x4, normal 4x4, mesh, override)]
+    lights: list = field(default_factory=list)      # [dict(LightDesc fields)]
+    ibl: tuple | None = None                        # (size, float32 faces (6,size,size,4))
+    sky: tuple = (4.0, 0.1, 1.15, 512)              # turbidity, albedo, normalized sun y, cube size
+    camera: dict = field(default_factory=dict)
+    width: int = 256
+    height: int = 256
+    max_ray_bounces: int = 5
+    spp: int = 64
+
+    @property
+    def num_triangles(self):
+        return int(sum(sum(s[1] for s in m.submeshes) // 3 for m in self.meshes))
+
+
+def material(albedo=(0.8, 0.8, 0.8, 1.0), emissive=(0, 0, 0, 0), metallic=0.0, roughness=1.0, albedo_texture=-1,
+             alpha_test=False, transparent=False, normal_texture=-1, roughness_texture=-1, roughness_channel=-1,
+             metallic_texture=-1, metallic_channel=-1, emissive_texture=-1):
+    return dict(albedo_value=tuple(float(x) for x in albedo), emissive_value=tuple(float(x) for x in emissive),
+                metallic_value=float(metallic), roughness_value=float(roughness), albedo_texture=albedo_texture,
+                normal_texture=normal_texture, roughness_texture=roughness_texture,
+                roughness_channel=roughness_channel, metallic_texture=metallic_texture,
+                metallic_channel=metallic_channel, emissive_texture=emissive_texture,
+                type=1 if transparent else 0, alpha_test=1 if alpha_test else 0)
+
+
+def linear_albedo(rgb):
+    """Albedo value whose packed form pow(v, 2.2) (scene.cpp:1017) is `rgb`."""
+    return tuple(float(c) ** (1.0 / 2.2) for c in rgb) + (1.0,)
+
+
+# ---------------------------------------------------------------------------
+# transforms / camera
+# ---------------------------------------------------------------------------
+def trs(translate=(0, 0, 0), rot=None, scale=1.0):
+    """Column-major (glm) model matrix T*R*S and normal matrix T*R."""
+    R = np.eye(4)
+    if rot is not None:
+        R[:3, :3] = rot
+    T = np.eye(4)
+    T[:3, 3] = translate
+    S = np.diag([scale, scale, scale, 1.0])
+    model = T @ R @ S
+    normal = T @ R
+    # glm is column-major: flatten the transpose of the row-major matrix
+    return model.T.astype(np.float32).ravel(), normal.T.astype(np.float32).ravel()
+
+
+def quat_from_matrix(m):
+    """(x, y, z, w) quaternion of a rotation matrix (rows = components)."""
+    t = m[0, 0] + m[1, 1] + m[2, 2]
+    if t > 0:
+        s = math.sqrt(t + 1.0) * 2
+        w, x, y, z = 0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        s = math.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+        w, x, y, z = (m[2, 1] - m[1, 2]) / s, 0.25 * s, (m[0, 1] + m[1, 0]) / s, (m[0, 2] + m[2, 0]) / s
+    elif m[1, 1] > m[2, 2]:
+        s = math.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+        w, x, y, z = (m[0, 2] - m[2, 0]) / s, (m[0, 1] + m[1, 0]) / s, 0.25 * s, (m[1, 2] + m[2, 1]) / s
+    else:
+        s = math.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+        w, x, y, z = (m[1, 0] - m[0, 1]) / s, (m[0, 2] + m[2, 0]) / s, (m[1, 2] + m[2, 1]) / s, 0.25 * s
+    q = np.array([x, y, z, w])
+    return tuple(float(c) for c in q / np.linalg.norm(q))
+
+
+def look_at_camera(eye, target, fov_y=60.0, near=1.0, far=1000.0, focal_length=8.0, aperture=0.0, up=(0, 1, 0)):
+    """CameraNode whose view direction (-(q*Z), path_integrator.cpp:138) points at target."""
+    eye = np.asarray(eye, float)
+    f = np.asarray(target, float) - eye
+    f /= np.linalg.norm(f)
+    r = np.cross(f, up)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    m = np.stack([r, u, -f], axis=1)  # columns: q*X, q*Y, q*Z
+    return dict(position=tuple(float(c) for c in eye), orientation=quat_from_matrix(m), fov_y_deg=float(fov_y),
+                near_plane=float(near), far_plane=float(far), focal_length=float(focal_length),
+                aperture_radius=float(aperture))
+
+
+def _orthonormal_tangents(n):
+    a = np.where(np.abs(n[:, 1:2]) < 0.99, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(a, n)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(n, t)
+    return t, b
+
+
+def make_vertices(pos, nrm, uv=None, submesh_index=None):
+    n = len(pos)
+    v = np.zeros(n, dtype=VERTEX_DTYPE)
+    v["position"][:, :3] = pos
+    if submesh_index is not None:
+        v["position"][:, 3] = submesh_index  # resource_manager.cpp:467-473
+    if uv is not None:
+        v["tex_coord"][:, :2] = uv
+    nrm = np.asarray(nrm, np.float64)
+    v["normal"][:, :3] = nrm
+    t, b = _orthonormal_tangents(nrm)
+    v["tangent"][:, :3] = t
+    v["bitangent"][:, :3] = b
+    return v
+
+
+def _quad(p0, p1, p2, p3, normal):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); returns positions, normals, uvs, indices."""
+    pos = np.array([p0, p1, p2, p3], np.float64)
+    nrm = np.tile(np.asarray(normal, np.float64), (4, 1))
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float64)
+    idx = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    return pos, nrm, uv, idx
+
+
+def _box(lo, hi):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    faces = [
+        ((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, 1)),
+        ((x1, y0, z0), (x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (0, 0, -1)),
+        ((x1, y0, z1), (x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (1, 0, 0)),
+        ((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-1, 0, 0)),
+        ((x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (x0, y1, z0), (0, 1, 0)),
+        ((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0)),
+    ]
+    P, N, U, I = [], [], [], []
+    for f in faces:
+        p, n, u, i = _quad(*f)
+        I.append(i + 4 * len(P))
+        P.append(p)
+        N.append(n)
+        U.append(u)
+    return np.concatenate(P), np.concatenate(N), np.concatenate(U), np.concatenate(I)
+
+
+def _mesh_from_parts(parts, materials):
+    """parts: [(pos, nrm, uv, idx, mat_idx)] -> MeshDesc with one submesh per part."""
+    P, N, U, I, S = [], [], [], [], []
+    vbase = 0
+    ibase = 0
+    for si, (p, n, u, i, m) in enumerate(parts):
+        v = make_vertices(p, n, u, submesh_index=si)
+        P.append(v)
+        I.append(i.astype(np.uint32) + vbase)
+        S.append((m, len(i), len(p), vbase, ibase))
+        vbase += len(p)
+        ibase += len(i)
+    return MeshDesc(np.concatenate(P), np.concatenate(I).astype(np.uint32), S, list(materials))
+
+
+def identity_instance(mesh, override=-1):
+    m, n = trs()
+    return (m, n, mesh, override)
+
+
+# ---------------------------------------------------------------------------
+# Cornell box
+# ---------------------------------------------------------------------------
+def cornell_box(seed=1, width=256, height=256, spp=64, max_ray_bounces=5):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = SceneDesc(width=width, height=height, spp=spp, max_ray_bounces=max_ray_bounces)
+    white = len(d.materials); d.materials.append(material(linear_albedo((0.75, 0.75, 0.75)), roughness=1.0))
+    red = len(d.materials); d.materials.append(material(linear_albedo((0.63, 0.06, 0.04)), roughness=1.0))
+    green = len(d.materials); d.materials.append(material(linear_albedo((0.14, 0.45, 0.09)), roughness=1.0))
+    light = len(d.materials); d.materials.append(
+        material(linear_albedo((0.75, 0.75, 0.75)), emissive=(17.0, 12.0, 4.0, 0.0), roughness=1.0))
+    # room: x in [-1,1], y in [0,2], z in [-1,1], open towards +z; normals face inwards
+    floor = _quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0))
+    ceil = _quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0))
+    back = _quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1))
+    left = _quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0))
+    right = _quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0))
+
+    def cat(*qs):
+        P, N, U, I = [], [], [], []
+        for q in qs:
+            I.append(q[3] + sum(len(p) for p in P))
+            P.append(q[0]); N.append(q[1]); U.append(q[2])
+        return np.concatenate(P), np.concatenate(N), np.concatenate(U), np.concatenate(I)
+
+    w = cat(floor, ceil, back)
+    room = _mesh_from_parts([(*w, 0), (*left, 1), (*right, 2)], [white, red, green])
+    # two axis-aligned boxes; the seed perturbs their footprints slightly
+    j = rng.uniform(-0.05, 0.05, size=4)
+    short = _box((0.10 + j[0], 0.0, -0.10 + j[1]), (0.70 + j[0], 0.60, 0.50 + j[1]))
+    tall = _box((-0.70 + j[2], 0.0, -0.70 + j[3]), (-0.10 + j[2], 1.20, -0.10 + j[3]))
+    boxes = _mesh_from_parts([cat(short, tall) + (0,)], [white])
+    lq = _quad((-0.25, 1.99, -0.25), (0.25, 1.99, -0.25), (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25), (0, -1, 0))
+    lmesh = _mesh_from_parts([(*lq, 0)], [light])
+    for m in (room, boxes, lmesh):
+        d.instances.append(identity_instance(len(d.meshes)))
+        d.meshes.append(m)
+    d.camera = look_at_camera((0.0, 1.0, 3.9), (0.0, 1.0, 0.0), fov_y=40.0, aperture=0.0)
+    return d
+
+
+# ---------------------------------------------------------------------------
+# Icosphere field
+# ---------------------------------------------------------------------------
+def icosphere(level):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    V = np.array(verts, np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    F = np.array(faces, np.int64)
+    for _ in range(level):
+        nv = len(V)
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+        e.sort(axis=1)
+        key = e[:, 0] * nv + e[:, 1]
+        uk, inv = np.unique(key, return_inverse=True)
+        a, b = uk // nv, uk % nv
+        mids = V[a] + V[b]
+        mids /= np.linalg.norm(mids, axis=1, keepdims=True)
+        m = inv.reshape(3, -1) + nv
+        m01, m12, m20 = m[0], m[1], m[2]
+        V = np.concatenate([V, mids])
+        F = np.concatenate([np.stack([F[:, 0], m01, m20], 1), np.stack([F[:, 1], m12, m01], 1),
+                            np.stack([F[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return V, F.astype(np.uint32)
+
+
+def _hash3(ix, iy, iz, seed):
+    h = (ix.astype(np.int64) * 73856093) ^ (iy.astype(np.int64) * 19349663) ^ (iz.astype(np.int64) * 83492791) \
+        ^ (seed * 2654435761)
+    h = (h ^ (h >> 13)) * 1274126177
+    h = h ^ (h >> 16)
+    return ((h & 0xFFFFFF).astype(np.float64) / float(0xFFFFFF)) * 2.0 - 1.0
+
+
+def value_noise(p, seed):
+    """Trilinear value noise with smoothstep fade, in [-1, 1]."""
+    f = np.floor(p)
+    i = f.astype(np.int64)
+    w = p - f
+    w = w * w * (3.0 - 2.0 * w)
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                c = _hash3(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed)
+                wx = w[:, 0] if dx else 1 - w[:, 0]
+                wy = w[:, 1] if dy else 1 - w[:, 1]
+                wz = w[:, 2] if dz else 1 - w[:, 2]
+                out = out + c * wx * wy * wz
+    return out
+
+
+def fbm3(p, seed, base_freq=2.0):
+    n = np.zeros(len(p))
+    amp, norm = 1.0, 0.0
+    for o in range(3):
+        n += amp * value_noise(p * (base_freq * 2 ** o), seed + 101 * o)
+        norm += amp
+        amp *= 0.5
+    return n / norm
+
+
+def vertex_normals(V, F):
+    fn = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    N = np.zeros_like(V)
+    for k in range(3):
+        np.add.at(N, F[:, k], fn)
+    N /= np.maximum(np.linalg.norm(N, axis=1, keepdims=True), 1e-30)
+    return N
+
+
+def displaced_sphere_mesh(level, seed, amplitude=0.05, mat=0):
+    V, F = icosphere(level)
+    r = 1.0 + amplitude * fbm3(V, seed)
+    P = V * r[:, None]
+    N = vertex_normals(P, F)
+    uv = np.stack([0.5 + np.arctan2(V[:, 2], V[:, 0]) / (2 * np.pi), 0.5 - np.arcsin(np.clip(V[:, 1], -1, 1)) / np.pi], 1)
+    return _mesh_from_parts([(P, N, uv, F.ravel(), 0)], [mat])
+
+
+def leaf_texture(size=128, seed=7):
+    """RGBA8 sRGB leaf: green colour, alpha = procedural leaf mask (~40 % opaque)."""
+    y, x = np.mgrid[0:size, 0:size].astype(np.float64)
+    u = (x + 0.5) / size * 2 - 1
+    v = (y + 0.5) / size * 2 - 1
+    # leaf outline: |u| < w(v), with w a lens profile, plus vein cut-outs
+    wv = 0.62 * np.sqrt(np.clip(1.0 - v * v, 0, 1)) * (0.9 + 0.1 * np.cos(7 * v))
+    inside = np.abs(u) < wv
+    veins = (np.abs(np.sin(12.0 * (v + 0.6 * np.abs(u)))) < 0.10) & (np.abs(u) > 0.06)
+    alpha = inside & ~veins
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = 0.55 + 0.25 * rng.random((size, size))
+    rgba = np.zeros((size, size, 4), np.uint8)
+    rgba[..., 0] = (0.25 * g * 255).astype(np.uint8)
+    rgba[..., 1] = (g * 255).astype(np.uint8)
+    rgba[..., 2] = (0.20 * g * 255).astype(np.uint8)
+    rgba[..., 3] = np.where(alpha, 255, 0).astype(np.uint8)
+    return rgba
+
+
+def sun_forward(elevation_deg, azimuth_deg):
+    """Directional-light forward (= -direction to the sun)."""
+    e, a = math.radians(elevation_deg), math.radians(azimuth_deg)
+    to_sun = np.array([math.cos(e) * math.cos(a), math.sin(e), math.cos(e) * math.sin(a)])
+    return tuple(float(c) for c in -to_sun)
+
+
+def icosphere_field(seed=0, n_spheres=64, level=5, n_leaves=2000, width=1920, height=1080, max_ray_bounces=8,
+                    ground=200.0, turbidity=3.0, sun_elevation=30.0, n_point_lights=4, spp=1):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = SceneDesc(width=width, height=height, spp=spp, max_ray_bounces=max_ray_bounces)
+    d.sky = (float(turbidity), 0.1, 1.15, 512)
+    half = ground / 2
+    # ground plane
+    gm = len(d.materials)
+    d.materials.append(material(linear_albedo((0.45, 0.42, 0.38)), roughness=0.9))
+    gq = _quad((-half, 0, half), (half, 0, half), (half, 0, -half), (-half, 0, -half), (0, 1, 0))
+    d.instances.append(identity_instance(len(d.meshes)))
+    d.meshes.append(_mesh_from_parts([(*gq, 0)], [gm]))
+    # spheres: one mesh (own BLAS) per seed, placed by an instance transform
+    placed = []
+    for i in range(n_spheres):
+        for _ in range(100):
+            radius = rng.uniform(3.0, 9.0)
+            c = rng.uniform(-half + 12, half - 12, size=2)
+            if all((c[0] - q[0]) ** 2 + (c[1] - q[1]) ** 2 > (radius + q[2] + 1.0) ** 2 for q in placed):
+                break
+        placed.append((c[0], c[1], radius))
+        mi = len(d.materials)
+        d.materials.append(material(tuple(rng.uniform(0, 1, 3)) + (1.0,), metallic=float(rng.random() < 0.3),
+                                    roughness=float(rng.uniform(0.05, 1.0))))
+        mesh = displaced_sphere_mesh(level, seed * 1000 + i, 0.05, mat=0)
+        mesh.materials = [mi]
+        ang = rng.uniform(0, 2 * np.pi)
+        rot = np.array([[math.cos(ang), 0, math.sin(ang)], [0, 1, 0], [-math.sin(ang), 0, math.cos(ang)]])
+        model, normal = trs((c[0], radius * 0.95, c[1]), rot, radius)
+        d.instances.append((model, normal, len(d.meshes), -1))
+        d.meshes.append(mesh)
+    # alpha-tested leaf quads (one mesh, one material, one texture)
+    tex = leaf_texture()
+    d.textures.append((tex.shape[1], tex.shape[0], TEXTURE_RGBA8_SRGB, tex))
+    lm = len(d.materials)
+    d.materials.append(material((1, 1, 1, 1), roughness=0.6, albedo_texture=0, alpha_test=True))
+    if n_leaves > 0:
+        ctr = np.stack([rng.uniform(-half + 5, half - 5, n_leaves), rng.uniform(0.5, 14.0, n_leaves),
+                        rng.uniform(-half + 5, half - 5, n_leaves)], 1)
+        size = rng.uniform(1.5, 4.0, n_leaves)
+        nrm = rng.normal(size=(n_leaves, 3))
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        t, b = _orthonormal_tangents(nrm)
+        hs = (size / 2)[:, None]
+        corners = np.stack([ctr - t * hs - b * hs, ctr + t * hs - b * hs, ctr + t * hs + b * hs, ctr - t * hs + b * hs], 1)
+        P = corners.reshape(-1, 3)
+        N = np.repeat(nrm, 4, axis=0)
+        U = np.tile(np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float64), (n_leaves, 1))
+        base = (np.arange(n_leaves, dtype=np.uint32) * 4)[:, None]
+        I = (base + np.array([0, 1, 2, 0, 2, 3], np.uint32)[None, :]).ravel()
+        d.instances.append(identity_instance(len(d.meshes)))
+        d.meshes.append(_mesh_from_parts([(P, N, U, I, 0)], [lm]))
+    # lights: the sun (directional) and point lights
+    d.lights.append(dict(type=LIGHT_DIRECTIONAL, color=(1.0, 0.96, 0.9), intensity=4.0, radius=0.1,
+                         position=(0.0, 0.0, 0.0), forward=sun_forward(sun_elevation, 35.0), inner_cone_deg=40.0,
+                         outer_cone_deg=50.0))
+    for k in range(n_point_lights):
+        p = (float(rng.uniform(-half * 0.6, half * 0.6)), float(rng.uniform(15, 30)),
+             float(rng.uniform(-half * 0.6, half * 0.6)))
+        col = tuple(float(c) for c in rng.uniform(0.6, 1.0, 3))
+        d.lights.append(dict(type=LIGHT_POINT, color=col, intensity=1500.0, radius=5.0, position=p,
+                             forward=(0.0, 0.0, 1.0), inner_cone_deg=40.0, outer_cone_deg=50.0))
+    d.camera = look_at_camera((0.0, 45.0, 150.0), (0.0, 0.0, 0.0), fov_y=60.0, near=1.0, far=1000.0, aperture=0.0)
+    return d
+
+
+def random_rays(n, lo, hi, seed=0, tmin=0.0, tmax=1e4):
+    """Rays from random points in the AABB [lo, hi] towards random directions."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    from .abi import RAY_DTYPE
+    r = np.zeros(n, RAY_DTYPE)
+    r["origin"] = rng.uniform(lo, hi, size=(n, 3))
+    dd = rng.normal(size=(n, 3))
+    r["direction"] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r

@@ -1,0 +1,342 @@
+/*
+ * helios_hip.h — C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * This library replaces Helios's Vulkan ray-tracing render path:
+ *   - helios::PathIntegrator            (include/gfx/path_integrator.h:9-62,
+ *                                        src/engine/gfx/path_integrator.cpp:48-200)
+ *   - the GPU packing of helios::Scene   (src/engine/resource/scene.cpp:915-1311)
+ *   - BLAS per mesh / TLAS build         (src/engine/resource/mesh.cpp:46-117,
+ *                                        src/engine/gfx/renderer.cpp:113-181)
+ *   - Hosek-Wilkie sky bake              (src/engine/gfx/hosek_wilkie_sky_model.cpp:658-764)
+ *   - output ping-pong / clear / tonemap (src/engine/gfx/renderer.cpp:183-223,369-430)
+ *
+ * Conventions
+ *   - Every entry point returns an hh_status; nothing throws across the ABI.
+ *     hh_last_error() returns a thread-local message for the last failure.
+ *   - Handles are opaque; inputs are copied at call time (the caller keeps its
+ *     host arrays); the library owns all device memory.
+ *   - A context is externally synchronized: one host thread at a time.
+ *   - Render calls are asynchronous on the context's HIP stream; hh_sync waits.
+ *   - Surfaces are RGBA32F, row-major, row 0 = NDC y = -1 (the reference's
+ *     storage-image orientation, renderer.cpp:396-399 flips it for display).
+ */
+#ifndef HELIOS_HIP_H
+#define HELIOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HH_ABI_VERSION 1
+
+/* Scene limits of the reference (include/resource/scene.h:14-17). */
+#define HH_MAX_SCENE_MESH_INSTANCE_COUNT 1024
+#define HH_MAX_SCENE_LIGHT_COUNT 100000
+#define HH_MAX_SCENE_MATERIAL_COUNT 4096
+#define HH_MAX_SCENE_MATERIAL_TEXTURE_COUNT (HH_MAX_SCENE_MATERIAL_COUNT * 4)
+
+typedef enum hh_status {
+    HH_OK = 0,
+    HH_ERR_INVALID_ARG = 1,
+    HH_ERR_OUT_OF_MEMORY = 2,
+    HH_ERR_LIMIT_EXCEEDED = 3,
+    HH_ERR_DEVICE = 4,
+    HH_ERR_NAN_DETECTED = 5,
+    HH_ERR_NOT_READY = 6
+} hh_status;
+
+typedef struct hh_context_t* hh_context;
+typedef struct hh_scene_t* hh_scene;
+typedef struct hh_surface_t* hh_surface;
+typedef struct hh_integrator_t* hh_integrator;
+
+/* ---------------------------------------------------------------------------
+ * Data layouts shared with the reference (byte compatible).
+ * ------------------------------------------------------------------------- */
+
+/* helios::Vertex, include/resource/mesh.h:10-17 (5 x vec4 = 80 B).
+ * position.w carries the submesh index as the reference loader writes it
+ * (src/engine/core/resource_manager.cpp:467-473). */
+typedef struct hh_vertex {
+    float position[4];
+    float tex_coord[4];
+    float normal[4];
+    float tangent[4];
+    float bitangent[4];
+} hh_vertex;
+
+/* helios::SubMesh, include/resource/mesh.h:19-29 (numeric fields only). */
+typedef struct hh_submesh {
+    uint32_t mat_idx;      /* index into the mesh's material list */
+    uint32_t index_count;
+    uint32_t vertex_count;
+    uint32_t base_vertex;
+    uint32_t base_index;
+} hh_submesh;
+
+/* helios::MaterialType, include/resource/material.h:12-16. */
+typedef enum hh_material_type { HH_MATERIAL_OPAQUE = 0, HH_MATERIAL_TRANSPARENT = 1 } hh_material_type;
+
+/* helios::Material (include/resource/material.h:25-99): host-side description.
+ * Texture ids come from hh_scene_add_texture, -1 = none. The packing into the
+ * 80-B GPU MaterialData record (src/engine/resource/scene.cpp:25-32,970-1109)
+ * happens at commit. */
+typedef struct hh_material_desc {
+    float albedo_value[4];    /* sRGB-ish; packed as pow(rgb, 2.2) (scene.cpp:1017) */
+    float emissive_value[4];
+    float metallic_value;
+    float roughness_value;
+    int32_t albedo_texture;
+    int32_t normal_texture;
+    int32_t roughness_texture;
+    int32_t roughness_channel;  /* roughness_texture_info().array_index, scene.cpp:1055 */
+    int32_t metallic_texture;
+    int32_t metallic_channel;   /* metallic_texture_info().array_index, scene.cpp:1076 */
+    int32_t emissive_texture;
+    uint32_t type;              /* hh_material_type */
+    uint32_t alpha_test;        /* is_alpha_tested() */
+} hh_material_desc;
+
+typedef enum hh_texture_format {
+    HH_TEXTURE_RGBA8_UNORM = 0,
+    HH_TEXTURE_RGBA8_SRGB = 1,  /* rgb decoded sRGB->linear per texel, alpha linear */
+    HH_TEXTURE_RGBA32F = 2
+} hh_texture_format;
+
+/* helios::MeshNode (include/resource/scene.h:143-160): one TLAS instance.
+ * model = global_transform() (column-major 4x4, glm layout); normal =
+ * normal_matrix() = global_transform_without_scale() (scene.cpp:277-280). */
+typedef struct hh_instance_desc {
+    float model[16];
+    float normal[16];
+    uint32_t mesh;               /* id from hh_scene_add_mesh */
+    int32_t material_override;   /* material id or -1 */
+} hh_instance_desc;
+
+/* Light node kinds; numeric values are the GPU light types (common.glsl:14-18). */
+typedef enum hh_light_type {
+    HH_LIGHT_DIRECTIONAL = 0,
+    HH_LIGHT_SPOT = 1,
+    HH_LIGHT_POINT = 2
+} hh_light_type;
+
+/* DirectionalLightNode / SpotLightNode / PointLightNode (scene.h:162-245).
+ * forward = orientation * +Z (scene.cpp:203-208). Defaults of the reference:
+ * radius 0.1 (directional) / 5 (spot, point); cone angles 40/50 degrees. */
+typedef struct hh_light_desc {
+    uint32_t type;          /* hh_light_type */
+    float color[3];
+    float intensity;
+    float radius;
+    float position[3];      /* global_position() for spot/point */
+    float forward[3];       /* forward() for directional/spot */
+    float inner_cone_deg;
+    float outer_cone_deg;
+} hh_light_desc;
+
+/* Hosek-Wilkie sky parameters. The reference hard-codes turbidity 4,
+ * ground albedo 0.1 and a sun luminance normalized to 1.15
+ * (include/gfx/hosek_wilkie_sky_model.h:32-34); they are settable here. */
+typedef struct hh_sky_params {
+    float turbidity;
+    float albedo;
+    float normalized_sun_y;
+    uint32_t cube_size;     /* 512 in the reference (hosek_wilkie_sky_model.cpp:17) */
+} hh_sky_params;
+
+/* CameraNode (include/resource/scene.h:247-282). orientation is a quaternion
+ * (x, y, z, w); position = global_position(); fov_y in degrees. */
+typedef struct hh_camera {
+    float position[3];
+    float orientation[4];
+    float fov_y_deg;
+    float near_plane;
+    float far_plane;
+    float focal_length;
+    float aperture_radius;
+} hh_camera;
+
+/* PathIntegrator settings (include/gfx/path_integrator.h:46-51) plus the
+ * extensions this build adds. Reference defaults: 7 bounces, 5000 samples,
+ * shadow bias 0, untiled. */
+typedef struct hh_settings {
+    uint32_t max_ray_bounces;
+    uint32_t max_samples;
+    float shadow_ray_bias;
+    uint32_t tiled;
+    uint32_t rr_start_depth;   /* 0 = reference behaviour: RR at every bounce */
+    uint32_t flags;            /* HH_FLAG_* */
+} hh_settings;
+
+#define HH_FLAG_NO_RADIANCE_CLAMP 0x1u   /* diagnostic: skip the min(L, 1) clamp */
+#define HH_FLAG_DIRECT_ONLY 0x2u         /* DIRECT_LIGHTING_INTEGRATOR variant */
+#define HH_FLAG_MEGAKERNEL 0x4u          /* one kernel per frame instead of the wavefront */
+
+/* The 192-B PathTraceConsts push-constant block (path_integrator.cpp:11-28). */
+typedef struct hh_launch_consts {
+    float view_proj_inverse[16];
+    float camera_pos[4];
+    float up_direction[4];
+    float right_direction[4];
+    float focal_plane[4];
+    int32_t ray_debug_pixel_coord[4];
+    uint32_t launch_id_size[4];
+    float accumulation;
+    uint32_t num_lights;
+    uint32_t num_frames;
+    uint32_t debug_vis;
+    uint32_t max_ray_bounces;
+    float shadow_ray_bias;
+    float focal_length;
+    float aperture_radius;
+} hh_launch_consts;
+
+/* A ray and a closest-hit record for the traversal entry points. */
+typedef struct hh_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} hh_ray;
+
+typedef struct hh_hit {
+    float t;           /* < 0 on miss */
+    float u, v;        /* barycentrics of v1, v2 (gl_HitAttribs) */
+    uint32_t instance; /* instanceCustomIndex = mesh-node index, 0xFFFFFFFF on miss */
+    uint32_t geometry; /* gl_GeometryIndexEXT = submesh */
+    uint32_t primitive;/* gl_PrimitiveID within the submesh */
+} hh_hit;
+
+#define HH_TRACE_OPAQUE 0x1u            /* gl_RayFlagsOpaqueEXT: skip any-hit */
+#define HH_TRACE_TERMINATE_ON_FIRST 0x4u
+
+/* ---------------------------------------------------------------------------
+ * Context
+ * ------------------------------------------------------------------------- */
+const char* hh_last_error(void);
+hh_status hh_context_create(int device_ordinal, hh_context* out);
+hh_status hh_context_destroy(hh_context ctx);
+hh_status hh_sync(hh_context ctx);
+/* HIP stream the context launches on (hipStream_t), for interop and events. */
+hh_status hh_context_stream(hh_context ctx, void** out_stream);
+
+/* ---------------------------------------------------------------------------
+ * Scene (replaces Scene + Mesh + Material + Texture + BatchUploader)
+ * ------------------------------------------------------------------------- */
+hh_status hh_scene_create(hh_context ctx, hh_scene* out);
+hh_status hh_scene_destroy(hh_scene scene);
+hh_status hh_scene_add_texture(hh_scene scene, uint32_t width, uint32_t height, uint32_t format,
+                               const void* texels, uint32_t* out_id);
+hh_status hh_scene_add_material(hh_scene scene, const hh_material_desc* desc, uint32_t* out_id);
+/* Mesh::create (mesh.cpp:27-42): vertices/indices/submeshes + the mesh's
+ * material list (material ids). */
+hh_status hh_scene_add_mesh(hh_scene scene, const hh_vertex* vertices, uint32_t num_vertices,
+                            const uint32_t* indices, uint32_t num_indices,
+                            const hh_submesh* submeshes, uint32_t num_submeshes,
+                            const uint32_t* material_ids, uint32_t num_materials, uint32_t* out_id);
+hh_status hh_scene_add_instance(hh_scene scene, const hh_instance_desc* desc, uint32_t* out_id);
+hh_status hh_scene_set_instance_transform(hh_scene scene, uint32_t instance, const float model[16],
+                                          const float normal[16]);
+hh_status hh_scene_add_light(hh_scene scene, const hh_light_desc* desc, uint32_t* out_id);
+/* IBLNode with a cube map: 6 faces of size x size RGBA32F in Vulkan layer
+ * order (+X,-X,+Y,-Y,+Z,-Z). */
+hh_status hh_scene_set_ibl(hh_scene scene, uint32_t size, const float* rgba32f_faces);
+hh_status hh_scene_set_sky(hh_scene scene, const hh_sky_params* params);
+/* Packs the GPU records, builds BLAS per mesh + TLAS on the GPU and bakes the
+ * sky. Any later edit marks the scene HIERARCHY_UPDATED and needs a commit. */
+hh_status hh_scene_commit(hh_scene scene);
+hh_status hh_scene_num_lights(hh_scene scene, uint32_t* out);
+/* Copies the packed light records (64 B each, scene.cpp:36-42) to host. */
+hh_status hh_scene_read_lights(hh_scene scene, float* out_records, uint32_t capacity);
+/* Copies the baked environment cube (6 x size x size RGBA32F) to host. */
+hh_status hh_scene_read_env(hh_scene scene, float* out_faces, uint32_t* out_size);
+/* Sky coefficients A..I, Z (10 x vec3, as float) used by the bake. */
+hh_status hh_scene_sky_coefficients(hh_scene scene, float out[30]);
+/* Host-only: the Hosek-Wilkie coefficient fit (A..I, Z as 10 x rgb) for a sun
+ * direction, as HosekWilkieSkyModel::update computes it on the CPU
+ * (hosek_wilkie_sky_model.cpp:658-701). Needs no device. */
+hh_status hh_sky_fit(const hh_sky_params* params, const float sun_dir[3], float out[30]);
+/* BVH statistics: node and triangle counts over all BLAS + TLAS. */
+hh_status hh_scene_bvh_stats(hh_scene scene, uint64_t out[4]);
+
+/* ---------------------------------------------------------------------------
+ * Output surface (replaces the RGBA32F ping-pong images)
+ * ------------------------------------------------------------------------- */
+hh_status hh_surface_create(hh_context ctx, uint32_t width, uint32_t height, hh_surface* out);
+hh_status hh_surface_destroy(hh_surface s);
+hh_status hh_surface_clear(hh_surface s);  /* vkCmdClearColorImage to (0,0,0,1) */
+hh_status hh_surface_read(hh_surface s, float* out_rgba);
+hh_status hh_surface_write(hh_surface s, const float* rgba);
+hh_status hh_surface_device_ptr(hh_surface s, void** out);
+/* tone_map.frag: exposure -> ACES (op 0) / Reinhard (op 1) / none -> gamma
+ * 1/2.2 -> RGBA8 (row 0 = bottom, as stored). */
+hh_status hh_tonemap(hh_surface s, float exposure, uint32_t op, uint8_t* out_rgba8);
+
+/* ---------------------------------------------------------------------------
+ * Integrator (replaces helios::PathIntegrator)
+ * ------------------------------------------------------------------------- */
+hh_status hh_integrator_create(hh_context ctx, hh_integrator* out);
+hh_status hh_integrator_destroy(hh_integrator it);
+hh_status hh_integrator_set_settings(hh_integrator it, const hh_settings* s);
+hh_status hh_integrator_get_settings(hh_integrator it, hh_settings* s);
+hh_status hh_integrator_restart_bake(hh_integrator it);
+/* PathIntegrator::render: one progressive sample (or one tile sample). */
+hh_status hh_integrator_render(hh_integrator it, hh_scene scene, const hh_camera* cam, hh_surface out);
+/* Renders `frames` consecutive progressive samples in one call (one hipGraph
+ * replay per frame). */
+hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh_camera* cam,
+                                      hh_surface out, uint32_t frames);
+/* counters: [num_accumulated_samples, num_target_samples, tile_idx, ...] */
+hh_status hh_integrator_counters(hh_integrator it, uint32_t out[4]);
+/* Path segments traced in the last render call: [extension rays incl.
+ * primary, shadow rays, frames]. */
+hh_status hh_integrator_ray_stats(hh_integrator it, uint64_t out[3]);
+/* PushConstants as launch_rays fills them (path_integrator.cpp:136-159). */
+hh_status hh_compute_launch_consts(const hh_camera* cam, uint32_t width, uint32_t height,
+                                   uint32_t num_lights, uint32_t num_frames,
+                                   const hh_settings* s, hh_launch_consts* out);
+
+/* ---------------------------------------------------------------------------
+ * Batched wavefront rendering: many views / scenes of uneven size in one
+ * persistent launch sequence per frame.
+ * ------------------------------------------------------------------------- */
+typedef struct hh_view {
+    hh_scene scene;
+    hh_surface surface;
+    hh_camera camera;
+    uint32_t spp;              /* progressive samples per call */
+    uint32_t max_ray_bounces;
+    float shadow_ray_bias;
+    uint32_t first_frame;      /* num_frames of the first sample */
+} hh_view;
+hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t num_views);
+
+/* ---------------------------------------------------------------------------
+ * Traversal entry points (closest / any hit with the reference's ray-flag
+ * semantics), used by parity tests.
+ * ------------------------------------------------------------------------- */
+hh_status hh_trace_rays(hh_scene scene, const hh_ray* rays, uint32_t n, uint32_t flags, hh_hit* out_hits);
+/* Primary rays of frame `num_frames` for every pixel, traced with flags 0. */
+hh_status hh_primary_hits(hh_scene scene, const hh_camera* cam, uint32_t width, uint32_t height,
+                          uint32_t num_frames, hh_hit* out_hits);
+
+/* ---------------------------------------------------------------------------
+ * Tuning: queue capacity, BVH width and workgroup shapes.
+ * ------------------------------------------------------------------------- */
+typedef struct hh_tuning {
+    uint32_t bvh_width;       /* 4 or 8 (BLAS/TLAS node width) */
+    uint32_t extend_block;    /* threads per block: 64/128/256 */
+    uint32_t shade_block;
+    uint32_t queue_capacity;  /* rays per wavefront pass; 0 = all paths */
+    uint32_t persistent_blocks_per_cu;
+} hh_tuning;
+hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
+hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HELIOS_HIP_H */

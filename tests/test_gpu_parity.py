@@ -1,0 +1,239 @@
+"""GPU parity: the HIP path against the CPU oracle on the same inputs.
+
+Bars (from the path's contract):
+  * primary-hit instance / geometry / primitive IDs bit-exact, except edge-tie
+    pixels at no more than 0.01 % of the image;
+  * traversal of arbitrary rays: same IDs and bit-identical t, u, v;
+  * radiance: relMSE (mean of (x - r)^2 / (r^2 + 0.01)) within 1e-3 of the
+    FP32 transcription and of the FP64 oracle (same per-pixel RNG streams);
+  * integer/byte outputs (light records, tonemapped RGBA8) exact.
+"""
+import numpy as np
+import pytest
+
+from helios_amd import scenegen
+from helios_amd.abi import HH_TRACE_OPAQUE, HH_TRACE_TERMINATE_ON_FIRST, HH_FLAG_MEGAKERNEL
+
+pytestmark = pytest.mark.gpu
+
+EDGE_TIE_FRACTION = 1e-4   # 0.01 % of pixels
+RELMSE_BAR = 1e-3
+
+
+def relmse(x, r, eps=0.01):
+    x = np.asarray(x, np.float64)[..., :3]
+    r = np.asarray(r, np.float64)[..., :3]
+    return float(np.mean((x - r) ** 2 / (r * r + eps)))
+
+
+def id_mismatch(a, b):
+    return (a["instance"] != b["instance"]) | (a["geometry"] != b["geometry"]) | (a["primitive"] != b["primitive"])
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    return scenegen.cornell_box()
+
+
+@pytest.fixture(scope="module")
+def small_field():
+    return scenegen.icosphere_field(seed=3, n_spheres=6, level=3, n_leaves=120, width=160, height=90,
+                                    max_ray_bounces=8)
+
+
+@pytest.fixture(scope="module")
+def cornell_pair(gpu_ctx, oracle_mod, cornell):
+    from helios_amd import runtime
+    return runtime.Scene(gpu_ctx, cornell), oracle_mod.OracleScene(cornell)
+
+
+@pytest.fixture(scope="module")
+def field_pair(gpu_ctx, oracle_mod, small_field):
+    from helios_amd import runtime
+    return runtime.Scene(gpu_ctx, small_field), oracle_mod.OracleScene(small_field)
+
+
+@pytest.mark.parametrize("frame", [0, 1, 7, 63])
+def test_primary_ids_cornell_bit_exact(cornell, cornell_pair, frame):
+    g, o = cornell_pair
+    hg = g.primary_hits(cornell.camera, cornell.width, cornell.height, frame)
+    ho = o.primary_hits(frame=frame, brute=True)
+    bad = id_mismatch(hg, ho)
+    assert bad.mean() <= EDGE_TIE_FRACTION, f"{bad.sum()} mismatching pixels"
+    both = ~bad & (ho["instance"] != 0xFFFFFFFF)
+    assert np.array_equal(hg["t"][both], ho["t"][both])
+    assert np.array_equal(hg["u"][both], ho["u"][both])
+    assert np.array_equal(hg["v"][both], ho["v"][both])
+
+
+def test_primary_ids_field_bit_exact(small_field, field_pair):
+    g, o = field_pair
+    d = small_field
+    for frame in (0, 5):
+        hg = g.primary_hits(d.camera, d.width, d.height, frame)
+        ho = o.primary_hits(frame=frame, brute=False)
+        bad = id_mismatch(hg, ho)
+        assert bad.mean() <= EDGE_TIE_FRACTION, f"frame {frame}: {bad.sum()} mismatching pixels"
+        both = ~bad & (ho["instance"] != 0xFFFFFFFF)
+        assert np.array_equal(hg["t"][both], ho["t"][both])
+
+
+@pytest.mark.parametrize("flags", [0, HH_TRACE_OPAQUE, HH_TRACE_OPAQUE | HH_TRACE_TERMINATE_ON_FIRST])
+def test_random_rays_cornell(cornell_pair, flags):
+    g, o = cornell_pair
+    rays = scenegen.random_rays(20000, (-0.99, 0.01, -0.99), (0.99, 1.98, 2.0), seed=11, tmin=1e-4)
+    hg = g.trace_rays(rays, flags)
+    ho = o.trace_rays(rays, flags, brute=True)
+    if flags & HH_TRACE_TERMINATE_ON_FIRST:
+        # any-hit: only the hit/miss decision is defined
+        assert np.array_equal(hg["instance"] == 0xFFFFFFFF, ho["instance"] == 0xFFFFFFFF)
+        return
+    assert not id_mismatch(hg, ho).any()
+    assert np.array_equal(hg["t"], ho["t"])
+    assert np.array_equal(hg["u"], ho["u"]) and np.array_equal(hg["v"], ho["v"])
+
+
+@pytest.mark.parametrize("flags", [0, HH_TRACE_OPAQUE])
+def test_random_rays_field_transforms_alpha(field_pair, flags):
+    g, o = field_pair
+    rays = scenegen.random_rays(30000, (-90, 0.5, -90), (90, 25, 90), seed=5, tmin=1e-4)
+    hg = g.trace_rays(rays, flags)
+    ho = o.trace_rays(rays, flags, brute=False)
+    bad = id_mismatch(hg, ho)
+    assert bad.mean() <= EDGE_TIE_FRACTION, f"{bad.sum()} mismatching rays"
+    ok = ~bad & (ho["instance"] != 0xFFFFFFFF)
+    assert np.array_equal(hg["t"][ok], ho["t"][ok])
+
+
+def test_scene_packing_matches(cornell_pair, field_pair):
+    for g, o in (cornell_pair, field_pair):
+        assert g.num_lights() == o.num_lights
+        assert np.array_equal(g.lights(), o.lights())
+
+
+def test_sky_bake_matches_oracle(field_pair):
+    g, o = field_pair
+    assert np.array_equal(g.sky_coefficients(), o.sky_coefficients())
+    eg, eo = g.env(), o.env()
+    assert eg.shape == eo.shape
+    rel = np.abs(eg[..., :3] - eo[..., :3]) / np.maximum(np.abs(eo[..., :3]), 1e-6)
+    assert rel.max() < 2e-5, rel.max()
+
+
+def _render_gpu(ctx, scene, desc, frames, first=0, flags=0, width=None, height=None, bounces=None, camera=None):
+    from helios_amd import runtime
+    w, h = width or desc.width, height or desc.height
+    surf = runtime.Surface(ctx, w, h)
+    it = runtime.PathIntegrator(ctx)
+    it.set_max_ray_bounces(bounces or desc.max_ray_bounces)
+    it.set_max_samples(100000)
+    it.set_flags(flags)
+    it.render(scene, camera or desc.camera, surf, frames=frames)
+    img = surf.read()
+    stats = it.ray_stats()
+    it.close()
+    surf.close()
+    return img, stats
+
+
+def test_radiance_cornell_matches_fp32_transcription(gpu_ctx, cornell, cornell_pair):
+    g, o = cornell_pair
+    img_g, _ = _render_gpu(gpu_ctx, g, cornell, frames=16)
+    img_o = o.render(frames=16)
+    assert relmse(img_g, img_o) < 1e-4
+    assert np.isfinite(img_g).all()
+
+
+def test_radiance_cornell_4096spp_crop(gpu_ctx, cornell, cornell_pair, oracle_mod):
+    """4096 progressive samples: GPU vs FP32 transcription and FP64 oracle on a 24x24 crop."""
+    g, o = cornell_pair
+    img_g, _ = _render_gpu(gpu_ctx, g, cornell, frames=4096)
+    crop = (116, 70, 140, 94)
+    x0, y0, x1, y1 = crop
+    img32 = o.render(frames=4096, crop=crop)
+    img64 = o.render(frames=4096, crop=crop, fp64=True)
+    cg = img_g[y0:y1, x0:x1]
+    assert relmse(cg, img32[y0:y1, x0:x1]) < RELMSE_BAR
+    assert relmse(cg, img64[y0:y1, x0:x1]) < RELMSE_BAR
+
+
+def test_radiance_field_matches_oracle(gpu_ctx, small_field, field_pair):
+    g, o = field_pair
+    img_g, _ = _render_gpu(gpu_ctx, g, small_field, frames=4)
+    img_o = o.render(frames=4)
+    assert relmse(img_g, img_o) < RELMSE_BAR
+
+
+def test_megakernel_equals_wavefront(gpu_ctx, small_field, field_pair):
+    g, _ = field_pair
+    a, sa = _render_gpu(gpu_ctx, g, small_field, frames=3)
+    b, sb = _render_gpu(gpu_ctx, g, small_field, frames=3, flags=HH_FLAG_MEGAKERNEL)
+    assert relmse(a, b) < 1e-7
+    assert sa["extension_rays"] == sb["extension_rays"]
+
+
+def test_tonemap_matches_oracle(gpu_ctx, oracle_mod):
+    from helios_amd import runtime
+    rng = np.random.default_rng(0)
+    img = (rng.random((64, 96, 4)) * np.array([4, 2, 1, 1])).astype(np.float32)
+    surf = runtime.Surface(gpu_ctx, 96, 64)
+    surf.write(img)
+    for op, exposure in ((0, 1.0), (1, 0.7), (2, 1.3)):
+        a = surf.tone_map(exposure, op)
+        b = oracle_mod.tonemap(img, exposure, op)
+        diff = np.abs(a.astype(int) - b.astype(int))
+        assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    surf.close()
+
+
+def test_progressive_protocol(gpu_ctx, cornell, cornell_pair):
+    """Frames rendered in separate calls accumulate exactly like one call."""
+    from helios_amd import runtime
+    g, _ = cornell_pair
+    a, _ = _render_gpu(gpu_ctx, g, cornell, frames=5)
+    surf = runtime.Surface(gpu_ctx, cornell.width, cornell.height)
+    it = runtime.PathIntegrator(gpu_ctx)
+    it.set_max_ray_bounces(cornell.max_ray_bounces)
+    for _ in range(5):
+        it.render(g, cornell.camera, surf)
+    assert it.num_accumulated_samples() == 5
+    b = surf.read()
+    assert np.array_equal(a, b)
+    it.restart_bake()
+    it.render(g, cornell.camera, surf)  # frame 0 overwrites
+    c = surf.read()
+    one, _ = _render_gpu(gpu_ctx, g, cornell, frames=1)
+    assert np.array_equal(c, one)
+
+
+def test_tiled_mode_matches_untiled(gpu_ctx, oracle_mod):
+    """Tiled progressive mode (128x128 tiles, max_samples each) ends at the same image."""
+    from helios_amd import runtime
+    d = scenegen.cornell_box(width=200, height=150)
+    g = runtime.Scene(gpu_ctx, d)
+    surf = runtime.Surface(gpu_ctx, 200, 150)
+    it = runtime.PathIntegrator(gpu_ctx)
+    it.set_max_ray_bounces(5)
+    it.set_max_samples(3)
+    it.set_tiled(True)
+    it.render(g, d.camera, surf, frames=3 * 4)
+    assert it.tile_idx() == 4
+    tiled = surf.read()
+    untiled, _ = _render_gpu(gpu_ctx, g, d, frames=3)
+    assert np.array_equal(tiled, untiled)
+
+
+def test_render_batch_matches_single_views(gpu_ctx, cornell, small_field, cornell_pair, field_pair):
+    from helios_amd import runtime
+    g1, _ = cornell_pair
+    g2, _ = field_pair
+    s1 = runtime.Surface(gpu_ctx, cornell.width, cornell.height)
+    s2 = runtime.Surface(gpu_ctx, small_field.width, small_field.height)
+    runtime.render_batch(gpu_ctx, [
+        dict(scene=g1, surface=s1, camera=cornell.camera, spp=3, max_ray_bounces=5),
+        dict(scene=g2, surface=s2, camera=small_field.camera, spp=2, max_ray_bounces=8),
+    ])
+    a1, _ = _render_gpu(gpu_ctx, g1, cornell, frames=3)
+    a2, _ = _render_gpu(gpu_ctx, g2, small_field, frames=2)
+    assert np.array_equal(s1.read(), a1)
+    assert np.array_equal(s2.read(), a2)
