@@ -79,6 +79,7 @@ struct hh_context_t {
     size_t dbg_cap = 0;
     float* d_coef = nullptr;
     bool use_graphs = true;
+    Prof prof;
 
     void drop_graphs() {
         for (auto& kv : graphs) {
@@ -345,6 +346,8 @@ hh_status hh_context_destroy(hh_context ctx) {
     if (ctx->d_views) (void)hipFree(ctx->d_views);
     if (ctx->dbg) (void)hipFree(ctx->dbg);
     if (ctx->d_coef) (void)hipFree(ctx->d_coef);
+    if (ctx->prof.d_counts) (void)hipFree(ctx->prof.d_counts);
+    for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HH_OK;
@@ -367,12 +370,62 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->bvh_width == 4 || t->bvh_width == 8, "bvh_width must be 4 or 8");
     HH_ARG(t->extend_block == 64 || t->extend_block == 128 || t->extend_block == 256, "extend_block must be 64/128/256");
     HH_ARG(t->persistent_blocks_per_cu >= 1 && t->persistent_blocks_per_cu <= 64, "blocks per CU must be 1..64");
+    HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 64, "frames_in_flight must be 1..64");
+    ctx->tune.frames_in_flight = t->frames_in_flight;
     ctx->tune.bvh_width = (int)t->bvh_width;
     ctx->tune.extend_block = (int)t->extend_block;
     ctx->tune.shade_block = (int)t->extend_block;
     ctx->tune.queue_capacity = t->queue_capacity;
     ctx->tune.blocks_per_cu = (int)t->persistent_blocks_per_cu;
     ctx->drop_graphs();
+    return HH_OK;
+}
+
+hh_status hh_context_set_profiling(hh_context ctx, uint32_t flags) {
+    HH_ARG(ctx, "null context");
+    HH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->prof.timing = (flags & 1u) != 0;
+    ctx->prof.count = (flags & 2u) != 0;
+    if (ctx->prof.count && !ctx->prof.d_counts) {
+        HH_HIP(hipMalloc(&ctx->prof.d_counts, 8 * sizeof(unsigned long long)));
+        HH_HIP(hipMemset(ctx->prof.d_counts, 0, 8 * sizeof(unsigned long long)));
+    }
+    return HH_OK;
+}
+
+hh_status hh_context_read_profile(hh_context ctx, double ms[6], uint64_t launches[6], uint64_t counts[6]) {
+    HH_ARG(ctx && ms && launches && counts, "null argument");
+    HH_HIP(hipStreamSynchronize(ctx->stream));
+    Prof& p = ctx->prof;
+    for (const Prof::Rec& r : p.recs) {
+        float t = 0.f;
+        HH_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        p.ms[r.kind] += t;
+        p.launches[r.kind]++;
+    }
+    p.recs.clear();
+    p.used = 0;
+    for (int k = 0; k < PK_COUNT; k++) {
+        ms[k] = p.ms[k];
+        launches[k] = p.launches[k];
+    }
+    unsigned long long c[8] = {0};
+    if (p.d_counts) HH_HIP(hipMemcpy(c, p.d_counts, sizeof(c), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; k++) counts[k] = c[k];
+    return HH_OK;
+}
+
+hh_status hh_context_reset_profile(hh_context ctx) {
+    HH_ARG(ctx, "null context");
+    HH_HIP(hipStreamSynchronize(ctx->stream));
+    Prof& p = ctx->prof;
+    p.recs.clear();
+    p.used = 0;
+    for (int k = 0; k < PK_COUNT; k++) {
+        p.ms[k] = 0;
+        p.launches[k] = 0;
+    }
+    if (p.d_counts) HH_HIP(hipMemset(p.d_counts, 0, 8 * sizeof(unsigned long long)));
     return HH_OK;
 }
 
@@ -383,6 +436,7 @@ hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t) {
     t->shade_block = (uint32_t)ctx->tune.shade_block;
     t->queue_capacity = ctx->tune.queue_capacity;
     t->persistent_blocks_per_cu = (uint32_t)ctx->tune.blocks_per_cu;
+    t->frames_in_flight = ctx->tune.frames_in_flight;
     return HH_OK;
 }
 
@@ -1006,11 +1060,20 @@ static hh_status run_frame(hh_context ctx, const std::vector<SceneGPU>& scenes, 
                            uint32_t total, uint32_t max_bounces, bool mega, const std::string& key_extra) {
     HH_HIP(ensure_tables(ctx, (uint32_t)scenes.size(), (uint32_t)views.size()));
     uint32_t cap_before = ctx->wf.capacity;
+    uint32_t* spill_before = ctx->wf.spill;
     HH_HIP(ctx->wf.ensure(total));
-    if (ctx->wf.capacity != cap_before) ctx->drop_graphs();
+    HH_HIP(ctx->wf.ensure_spill((size_t)ctx->num_cu * ctx->tune.blocks_per_cu * ctx->tune.extend_block));
+    if (ctx->wf.capacity != cap_before || ctx->wf.spill != spill_before) ctx->drop_graphs();
     HH_HIP(hipMemcpyAsync(ctx->d_scene_table, scenes.data(), scenes.size() * sizeof(SceneGPU), hipMemcpyHostToDevice, ctx->stream));
     HH_HIP(hipMemcpyAsync(ctx->d_views, views.data(), views.size() * sizeof(ViewGPU), hipMemcpyHostToDevice, ctx->stream));
     LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
+    if (ctx->prof.timing || ctx->prof.count) {
+        // profiled frames are launched directly so every kernel gets its events
+        cfg.prof = &ctx->prof;
+        HH_HIP(launch_frame(cfg, ctx->d_scene_table, scenes.data(), ctx->d_views, views.data(), (uint32_t)views.size(), total,
+                            max_bounces, ctx->wf, mega));
+        return HH_OK;
+    }
     if (!ctx->use_graphs) {
         HH_HIP(launch_frame(cfg, ctx->d_scene_table, scenes.data(), ctx->d_views, views.data(), (uint32_t)views.size(), total,
                             max_bounces, ctx->wf, mega));
@@ -1114,36 +1177,47 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
         it->last_generation = scene->generation;
         it->last_scene = scene;
     }
+    // size the wavefront for the largest chunk first, so the statistics
+    // counters are not reallocated after they are cleared
+    const uint32_t max_chunk = std::max<uint32_t>(1, std::min<uint32_t>(frames, ctx->tune.frames_in_flight));
+    HH_HIP(ctx->wf.ensure(out->w * out->h * max_chunk));
     unsigned long long zero[2] = {0, 0};
-    HH_HIP(ctx->wf.ensure(out->w * out->h));
     HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
     const bool mega = (it->s.flags & HH_FLAG_MEGAKERNEL) != 0;
     std::vector<SceneGPU> scenes{scene->gpu};
     uint32_t launched = 0;
-    for (uint32_t f = 0; f < frames; f++) {
+    // Frames are traced in chunks of up to frames_in_flight progressive
+    // samples per wavefront pass (never crossing a tile's max_samples).
+    uint32_t f = 0;
+    while (f < frames) {
         if (it->num_acc == 0 && it->tile_idx == 0) HH_HIP(launch_fill_f4(ctx->stream, out->d, make_float4(0.f, 0.f, 0.f, 1.f), out->w * out->h));
-        if (it->tile_idx < it->tiles.size()) {
-            ViewGPU v = make_view(*cam, out->w, out->h);
-            auto t = it->tiles[it->tile_idx];
-            v.tile_x = t.first;
-            v.tile_y = t.second;
-            v.tile_w = it->s.tiled ? std::min<uint32_t>(128, out->w - t.first) : out->w;
-            v.tile_h = it->s.tiled ? std::min<uint32_t>(128, out->h - t.second) : out->h;
-            v.spp = 1;
-            v.first_frame = it->num_acc;
-            v.max_bounces = it->s.max_ray_bounces;
-            v.shadow_bias = it->s.shadow_ray_bias;
-            v.flags = it->s.flags;
-            v.scene = 0;
-            v.slot_base = 0;
-            v.accum = out->d;
-            std::vector<ViewGPU> views{v};
-            st = run_frame(ctx, scenes, views, v.tile_w * v.tile_h, it->s.max_ray_bounces, mega,
-                           std::to_string(scene->generation) + "|" + std::to_string((uintptr_t)out->d));
-            if (st != HH_OK) return st;
-            it->num_acc++;
-            launched++;
+        if (it->tile_idx >= it->tiles.size()) {
+            f++;
+            continue;
         }
+        uint32_t chunk = std::min(frames - f, std::max<uint32_t>(1, ctx->tune.frames_in_flight));
+        chunk = std::min(chunk, it->s.max_samples - it->num_acc);
+        ViewGPU v = make_view(*cam, out->w, out->h);
+        auto t = it->tiles[it->tile_idx];
+        v.tile_x = t.first;
+        v.tile_y = t.second;
+        v.tile_w = it->s.tiled ? std::min<uint32_t>(128, out->w - t.first) : out->w;
+        v.tile_h = it->s.tiled ? std::min<uint32_t>(128, out->h - t.second) : out->h;
+        v.spp = chunk;
+        v.first_frame = it->num_acc;
+        v.max_bounces = it->s.max_ray_bounces;
+        v.shadow_bias = it->s.shadow_ray_bias;
+        v.flags = it->s.flags;
+        v.scene = 0;
+        v.slot_base = 0;
+        v.accum = out->d;
+        std::vector<ViewGPU> views{v};
+        st = run_frame(ctx, scenes, views, v.tile_w * v.tile_h * chunk, it->s.max_ray_bounces, mega,
+                       std::to_string(scene->generation) + "|" + std::to_string((uintptr_t)out->d));
+        if (st != HH_OK) return st;
+        it->num_acc += chunk;
+        launched += chunk;
+        f += chunk;
         if (it->num_acc == it->s.max_samples) {
             it->num_acc = 0;
             it->tile_idx++;
@@ -1290,7 +1364,9 @@ hh_status hh_trace_rays(hh_scene s, const hh_ray* rays, uint32_t n, uint32_t fla
     uint32_t* d_ids = (uint32_t*)(d_hit + n);
     HH_HIP(hipMemcpyAsync(d_rays, rays, size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream));
     LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
-    HH_HIP(launch_trace_rays(cfg, s->d_gpu, s->gpu, d_rays, n, flags, d_hit, d_ids));
+    HH_HIP(ctx->wf.ensure_spill(((size_t)n + 255) / 256 * 256));
+    ctx->drop_graphs();
+    HH_HIP(launch_trace_rays(cfg, s->d_gpu, s->gpu, d_rays, n, flags, d_hit, d_ids, ctx->wf.spill));
     std::vector<float4> hit(n);
     std::vector<uint32_t> ids(2 * size_t(n));
     HH_HIP(hipMemcpyAsync(hit.data(), d_hit, size_t(n) * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1314,7 +1390,9 @@ hh_status hh_primary_hits(hh_scene s, const hh_camera* cam, uint32_t w, uint32_t
     ViewGPU v = make_view(*cam, w, h);
     HH_HIP(hipMemcpyAsync(d_view, &v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
     LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
-    HH_HIP(launch_primary_hits(cfg, s->d_gpu, s->gpu, d_view, w, h, frame, d_hit, d_ids));
+    HH_HIP(ctx->wf.ensure_spill((n + 255) / 256 * 256));
+    ctx->drop_graphs();
+    HH_HIP(launch_primary_hits(cfg, s->d_gpu, s->gpu, d_view, w, h, frame, d_hit, d_ids, ctx->wf.spill));
     std::vector<float4> hit(n);
     std::vector<uint32_t> ids(2 * n);
     HH_HIP(hipMemcpyAsync(hit.data(), d_hit, n * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1353,7 +1431,20 @@ hipError_t hh::Wavefront::ensure(uint32_t n) {
     return hipSuccess;
 }
 
+hipError_t hh::Wavefront::ensure_spill(size_t threads) {
+    if (threads <= spill_threads && spill) return hipSuccess;
+    if (spill) HH_TRY(hipFree(spill));
+    spill = nullptr;
+    spill_threads = 0;
+    HH_TRY(hipMalloc(&spill, threads * kSpillStackDepth * sizeof(uint32_t)));
+    spill_threads = threads;
+    return hipSuccess;
+}
+
 void hh::Wavefront::release() {
+    if (spill) (void)hipFree(spill);
+    spill = nullptr;
+    spill_threads = 0;
     void* ps[] = {T, L, rng, q_o[0], q_o[1], q_d[0], q_d[1], q_slot[0], q_slot[1], hit, hit_tri, s_o, s_d, s_c, s_slot, counters, stats};
     for (void* p : ps)
         if (p) (void)hipFree(p);

@@ -30,6 +30,9 @@ constexpr uint32_t HH_FLAG_MEGAKERNEL_BIT = 0x4u;
 //                            27..30 count-1 (1..16 triangles); in the TLAS the
 //                            "first triangle" is the instance index.
 //   0xFFFFFFFF            -> stack marker "leave the BLAS, restore world ray"
+// Traversal stack entries kept outside LDS, per launched thread.
+constexpr int kSpillStackDepth = 104;
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMarker = 0xFFFFFFFFu;
 constexpr uint32_t kLeafStartMask = 0x07FFFFFFu;

@@ -322,26 +322,45 @@ __global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ vie
     }
 }
 
-template <int W, int BS>
+// Adds per-lane traversal counts into three device counters (one atomic per wave).
+__device__ inline void flush_counts(const TravCount& tc, unsigned long long* out) {
+    unsigned long long a = tc.nodes, b = tc.tris, c = tc.insts;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&out[0], a);
+        atomicAdd(&out[1], b);
+        atomicAdd(&out[2], c);
+    }
+}
+
+template <int W, int BS, bool COUNT>
 __global__ void __launch_bounds__(BS) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                                const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count_p,
                                                uint32_t rflags, float4* __restrict__ hit, uint32_t* __restrict__ hit_tri,
-                                               unsigned long long* __restrict__ stats) {
+                                               unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
+                                               uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
+    const LaneStack stk{&stack[threadIdx.x], BS, spill + blockIdx.x * BS + threadIdx.x, gridDim.x * BS};
     const uint32_t n = *count_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)n);
     const uint32_t stride = gridDim.x * BS;
+    TravCount tc{0u, 0u, 0u};
     for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n; q += stride) {
         uint32_t slot = q_slot[q];
         uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
         const SceneGPU& S = scenes[views[view].scene];
         float4 o = q_o[q], d = q_d[q];
         HitRec h;
-        trace_ray<W, false>(S, ld3(o), ld3(d), o.w, d.w, rflags, h, &stack[threadIdx.x], BS);
+        trace_ray<W, false, COUNT>(S, ld3(o), ld3(d), o.w, d.w, rflags, h, stk, &tc);
         hit[q] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.inst));
         hit_tri[q] = h.found ? h.tri : 0xFFFFFFFFu;
     }
+    if (COUNT) flush_counts(tc, tcounts);
 }
 
 template <int BS>
@@ -425,28 +444,32 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
     }
 }
 
-template <int W, int BS>
+template <int W, int BS, bool COUNT>
 __global__ void __launch_bounds__(BS) k_shadow(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ s_o, const float4* __restrict__ s_d,
                                                const float4* __restrict__ s_c, const uint32_t* __restrict__ s_slot,
                                                const uint32_t* __restrict__ count_p, float4* __restrict__ L,
-                                               unsigned long long* __restrict__ stats) {
+                                               unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
+                                               uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
+    const LaneStack stk{&stack[threadIdx.x], BS, spill + blockIdx.x * BS + threadIdx.x, gridDim.x * BS};
     const uint32_t n = *count_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], (unsigned long long)n);
     const uint32_t stride = gridDim.x * BS;
+    TravCount tc{0u, 0u, 0u};
     for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n; q += stride) {
         uint32_t slot = s_slot[q];
         uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
         const SceneGPU& S = scenes[views[view].scene];
         float4 o = s_o[q], d = s_d[q], c = s_c[q];
         HitRec h;
-        bool occluded = trace_ray<W, true>(S, ld3(o), ld3(d), o.w, d.w, __float_as_uint(c.w), h, &stack[threadIdx.x], BS);
+        bool occluded = trace_ray<W, true, COUNT>(S, ld3(o), ld3(d), o.w, d.w, __float_as_uint(c.w), h, stk, &tc);
         if (!occluded) {
             float4 l = L[slot];
             L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
         }
     }
+    if (COUNT) flush_counts(tc, tcounts + 3);
 }
 
 __global__ void k_reset2(uint32_t* a, uint32_t* b) {
@@ -484,8 +507,9 @@ __global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, con
 template <int W, int BS>
 __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
                                                    uint32_t nv, uint32_t total, float4* __restrict__ Lout,
-                                                   unsigned long long* __restrict__ stats) {
+                                                   unsigned long long* __restrict__ stats, uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
+    const LaneStack stk{&stack[threadIdx.x], BS, spill + blockIdx.x * BS + threadIdx.x, gridDim.x * BS};
     const uint32_t stride = gridDim.x * BS;
     unsigned long long n_ext = 0, n_sh = 0;
     for (uint32_t slot = blockIdx.x * BS + threadIdx.x; slot < total; slot += stride) {
@@ -501,7 +525,7 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
         for (uint32_t depth = 0;; depth++) {
             HitRec h;
             n_ext++;
-            trace_ray<W, false>(S, o, d, tmin, 10000.0f, rflags, h, &stack[threadIdx.x], BS);
+            trace_ray<W, false>(S, o, d, tmin, 10000.0f, rflags, h, stk);
             if (!h.found) {
                 f3 e = sample_env(S, d);
                 Lp = Lp + (depth == 0 ? e : T * e);
@@ -514,7 +538,7 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
             if (sr.want) {
                 HitRec sh;
                 n_sh++;
-                if (!trace_ray<W, true>(S, sr.o, sr.d, sr.tmin, sr.tmax, sr.flags, sh, &stack[threadIdx.x], BS))
+                if (!trace_ray<W, true>(S, sr.o, sr.d, sr.tmin, sr.tmax, sr.flags, sh, stk))
                     Lp = Lp + sr.contrib;
             }
             if ((V.flags & HH_FLAG_DIRECT_ONLY_BIT) != 0u || depth + 1 >= V.max_bounces) break;
@@ -546,17 +570,19 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
 // Debug / parity entry points ------------------------------------------------
 template <int W>
 __global__ void __launch_bounds__(256) k_trace_rays(const SceneGPU* __restrict__ scene, const float4* __restrict__ rays, uint32_t n,
-                                                    uint32_t flags, float4* __restrict__ out_hit, uint32_t* __restrict__ out_ids) {
+                                                    uint32_t flags, float4* __restrict__ out_hit, uint32_t* __restrict__ out_ids,
+                                                    uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * 256];
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const LaneStack stk{&stack[threadIdx.x], 256, spill + i, gridDim.x * 256};
     if (i >= n) return;
     const SceneGPU& S = *scene;
     float4 o = rays[2 * i], d = rays[2 * i + 1];
     HitRec h;
     if (flags & HH_TRACE_TERMINATE_BIT)
-        trace_ray<W, true>(S, ld3(o), ld3(d), o.w, d.w, flags, h, &stack[threadIdx.x], 256);
+        trace_ray<W, true>(S, ld3(o), ld3(d), o.w, d.w, flags, h, stk);
     else
-        trace_ray<W, false>(S, ld3(o), ld3(d), o.w, d.w, flags, h, &stack[threadIdx.x], 256);
+        trace_ray<W, false>(S, ld3(o), ld3(d), o.w, d.w, flags, h, stk);
     out_hit[i] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.found ? h.inst : 0xFFFFFFFFu));
     uint32_t geom = 0xFFFFFFFFu, prim = 0xFFFFFFFFu;
     if (h.found) {
@@ -570,9 +596,10 @@ __global__ void __launch_bounds__(256) k_trace_rays(const SceneGPU* __restrict__
 template <int W>
 __global__ void __launch_bounds__(256) k_primary(const SceneGPU* __restrict__ scene, const ViewGPU* __restrict__ view, uint32_t w,
                                                  uint32_t h_, uint32_t frame, float4* __restrict__ out_hit,
-                                                 uint32_t* __restrict__ out_ids) {
+                                                 uint32_t* __restrict__ out_ids, uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * 256];
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const LaneStack stk{&stack[threadIdx.x], 256, spill + i, gridDim.x * 256};
     if (i >= w * h_) return;
     uint32_t x = i % w, y = i / w;
     const SceneGPU& S = *scene;
@@ -580,7 +607,7 @@ __global__ void __launch_bounds__(256) k_primary(const SceneGPU* __restrict__ sc
     f3 o, d;
     generate_ray(*view, x, y, rng, o, d);
     HitRec h;
-    trace_ray<W, false>(S, o, d, 0.001f, 10000.0f, 0u, h, &stack[threadIdx.x], 256);
+    trace_ray<W, false>(S, o, d, 0.001f, 10000.0f, 0u, h, stk);
     out_hit[i] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.found ? h.inst : 0xFFFFFFFFu));
     uint32_t geom = 0xFFFFFFFFu, prim = 0xFFFFFFFFu;
     if (h.found) {
@@ -652,32 +679,75 @@ __global__ void k_fill_f4(float4* p, float4 v, uint32_t n) {
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-template <int W, int BS>
+// Brackets one launch with HIP events on the launch stream when profiling.
+struct ProfScope {
+    Prof* p;
+    int kind;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    ProfScope(Prof* p_, int k, hipStream_t s) : p(p_), kind(k), st(s) {
+        if (p && p->timing) {
+            a = p->ev();
+            if (a) (void)hipEventRecord(a, st);
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            hipEvent_t b = p->ev();
+            if (b) {
+                (void)hipEventRecord(b, st);
+                p->recs.push_back({kind, a, b});
+            }
+        }
+    }
+};
+
+template <int W, int BS, bool COUNT>
 static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, const ViewGPU* d_views, const ViewGPU* h_views,
                              uint32_t nv, uint32_t total, uint32_t max_bounces, Wavefront& wf, bool mega) {
     hipStream_t st = cfg.stream;
+    Prof* pf = cfg.prof;
+    unsigned long long* tc = pf ? pf->d_counts : nullptr;
     const uint32_t grid_cap = (uint32_t)cfg.num_cu * (uint32_t)cfg.tune.blocks_per_cu;
     const uint32_t grid = std::max(1u, std::min(grid_cap, cdiv(total, BS)));
     if (mega) {
-        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats);
+        ProfScope ps(pf, PK_MEGAKERNEL, st);
+        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats, wf.spill);
     } else {
-        k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, wf.counters);
+        {
+            ProfScope ps(pf, PK_GENERATE, st);
+            k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng,
+                                                wf.counters);
+        }
         for (uint32_t d = 0; d < max_bounces; d++) {
             const int a = d & 1, b = a ^ 1;
-            k_extend<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a], wf.counters + a,
-                                                 d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit, wf.hit_tri, wf.stats);
+            {
+                ProfScope ps(pf, PK_EXTEND, st);
+                k_extend<W, BS, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
+                                                            wf.counters + a, d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
+                                                            wf.hit_tri, wf.stats, tc, wf.spill);
+            }
             k_reset2<<<1, 64, 0, st>>>(wf.counters + b, wf.counters + 2);
-            k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], wf.counters + a, wf.hit,
-                                             wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b], wf.counters + b,
-                                             wf.s_o, wf.s_d, wf.s_c, wf.s_slot, wf.counters + 2);
-            k_shadow<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, wf.counters + 2, wf.L,
-                                                 wf.stats);
+            {
+                ProfScope ps(pf, PK_SHADE, st);
+                k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], wf.counters + a,
+                                                 wf.hit, wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b],
+                                                 wf.counters + b, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, wf.counters + 2);
+            }
+            {
+                ProfScope ps(pf, PK_SHADOW, st);
+                k_shadow<W, BS, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+                                                            wf.counters + 2, wf.L, wf.stats, tc, wf.spill);
+            }
         }
     }
     uint32_t maxpix = 0;
     for (uint32_t i = 0; i < nv; i++) maxpix = std::max(maxpix, h_views[i].tile_w * h_views[i].tile_h);
     dim3 ag(std::max(1u, std::min(cdiv(maxpix, 256), 1024u)), nv);
-    k_accumulate<<<ag, 256, 0, st>>>(d_views, nv, wf.L);
+    {
+        ProfScope ps(pf, PK_ACCUMULATE, st);
+        k_accumulate<<<ag, 256, 0, st>>>(d_views, nv, wf.L);
+    }
     return hipGetLastError();
 }
 
@@ -686,7 +756,11 @@ hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const Sc
                         bool mega) {
     const int W = h_scenes[0].bvh_width;
     const int bs = cfg.tune.extend_block;
-#define HH_DISPATCH(WW, BB) return frame_impl<WW, BB>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
+    if (cfg.prof && cfg.prof->count && cfg.prof->d_counts) {
+        if (W == 8) return frame_impl<8, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
+        return frame_impl<4, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
+    }
+#define HH_DISPATCH(WW, BB) return frame_impl<WW, BB, false>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
     if (W == 8) {
         if (bs == 64) HH_DISPATCH(8, 64);
         if (bs == 128) HH_DISPATCH(8, 128);
@@ -699,18 +773,18 @@ hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const Sc
 }
 
 hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,
-                             uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_ids) {
+                             uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_ids, uint32_t* spill) {
     if (n == 0) return hipSuccess;
-    if (h_scene.bvh_width == 8) k_trace_rays<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids);
-    else k_trace_rays<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids);
+    if (h_scene.bvh_width == 8) k_trace_rays<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
+    else k_trace_rays<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
     return hipGetLastError();
 }
 
 hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
-                               uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_ids) {
+                               uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_ids, uint32_t* spill) {
     uint32_t n = w * h;
-    if (h_scene.bvh_width == 8) k_primary<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids);
-    else k_primary<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids);
+    if (h_scene.bvh_width == 8) k_primary<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
+    else k_primary<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
     return hipGetLastError();
 }
 

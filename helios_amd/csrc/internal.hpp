@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "gpu_scene.hpp"
 
 #define HH_TRY(expr)                             \
@@ -56,7 +58,11 @@ struct Wavefront {
     // counters: [0,1] extension queue sizes, [2] shadow size, [3..] stats
     uint32_t* counters = nullptr;
     unsigned long long* stats = nullptr;  // [0] extension rays, [1] shadow rays
+    // traversal stack overflow, kSpillStackDepth words per launched thread
+    uint32_t* spill = nullptr;
+    size_t spill_threads = 0;
     hipError_t ensure(uint32_t n);
+    hipError_t ensure_spill(size_t threads);
     void release();
     ~Wavefront() { release(); }
 };
@@ -68,6 +74,33 @@ struct Tuning {
     uint32_t queue_capacity = 0;
     int blocks_per_cu = 8;
     int leaf_size = 4;
+    uint32_t frames_in_flight = 4;
+};
+
+// Optional per-kernel profiling of a frame: HIP events around every launch
+// (timing) and the counting traversal variant (algorithmic bytes).
+enum ProfKind { PK_GENERATE = 0, PK_EXTEND, PK_SHADE, PK_SHADOW, PK_ACCUMULATE, PK_MEGAKERNEL, PK_COUNT };
+struct Prof {
+    bool timing = false;
+    bool count = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    double ms[PK_COUNT] = {0};
+    unsigned long long launches[PK_COUNT] = {0};
+    unsigned long long* d_counts = nullptr;  // [ext nodes, ext tris, ext insts, shadow nodes, shadow tris, shadow insts]
+    hipEvent_t ev() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
 };
 
 // Kernel launchers (integrator.hip)
@@ -75,14 +108,15 @@ struct LaunchCfg {
     hipStream_t stream;
     int num_cu;
     Tuning tune;
+    Prof* prof = nullptr;
 };
 hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const SceneGPU* h_scenes, const ViewGPU* d_views,
                         const ViewGPU* h_views, uint32_t num_views, uint32_t total_paths, uint32_t max_bounces,
                         Wavefront& wf, bool megakernel);
 hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,
-                             uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_hit_ids);
+                             uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
 hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
-                               uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_hit_ids);
+                               uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
 hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst);
 hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const float* d_coef, const float sun[3]);
 

@@ -9,8 +9,8 @@
 //  src/engine/resource/scene.cpp:1257).
 //
 // Per lane: a short stack of kLdsStack entries in LDS (interleaved by lane so
-// the 64 lanes of a wave hit 64 different banks) that spills into a private
-// array when deeper. Boxes use the robust slab test of Ize ("Robust BVH Ray
+// the 64 lanes of a wave hit 64 different banks) that spills into a global
+// per-thread area when deeper (no private/scratch memory). Boxes use the robust slab test of Ize ("Robust BVH Ray
 // Traversal", JCGT 2013); triangles use the watertight test of Woop, Benthin
 // and Wald (JCGT 2013) with its double-precision fallback on zero edge
 // functions. Equal-t hits are resolved to the smallest (instance, geometry,
@@ -22,8 +22,24 @@
 
 namespace hh {
 
-constexpr int kLdsStack = 16;
-constexpr int kSpillStack = 112;
+constexpr int kLdsStack = 24;
+constexpr int kSpillStack = kSpillStackDepth;
+
+// Where one lane keeps its traversal stack: the first kLdsStack entries in LDS
+// (entry k at lds[k * lds_stride]), deeper entries in a global per-thread
+// spill area (entry k at spill[(k - kLdsStack) * spill_stride]). Keeping the
+// spill out of private memory keeps the kernels free of scratch.
+struct LaneStack {
+    uint32_t* lds;
+    uint32_t lds_stride;
+    uint32_t* spill;
+    uint32_t spill_stride;
+};
+
+// Per-ray fetch counts of the counting variant (algorithmic bytes).
+struct TravCount {
+    uint32_t nodes, tris, insts;
+};
 
 struct HitRec {
     float t, u, v;
@@ -152,20 +168,20 @@ __device__ inline bool key_less(uint32_t i1, uint32_t g1, uint32_t p1, uint32_t 
 // ---------------------------------------------------------------------------
 // Traversal. ANY = stop at the first accepted hit (visibility rays).
 // rflags: HH_TRACE_OPAQUE (bit 0) disables the any-hit alpha test.
-// `lds` points at this lane's first stack word; consecutive entries are
-// `stride` words apart.
+// `stk` says where this lane's stack lives.
 // ---------------------------------------------------------------------------
-template <int W, bool ANY>
+template <int W, bool ANY, bool COUNT = false>
 __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, float tmax, uint32_t rflags, HitRec& h,
-                                 uint32_t* lds, int stride) {
+                                 const LaneStack& stk, TravCount* cnt = nullptr) {
     const WideNode<W>* nodes = (const WideNode<W>*)S.nodes;
     const float4* tris = S.tris;
+    uint32_t* const lds = stk.lds;
+    const uint32_t stride = stk.lds_stride;
     f3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
     f3 o = wo, inv = winv;
     WT wt = wt_pre(wd);
     int inst = -1;
     uint32_t sp = 0;
-    uint32_t spill[kSpillStack];
     h.found = false;
     h.t = tmax;
     h.u = h.v = 0.0f;
@@ -177,7 +193,7 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
     // bounds); the capacity is far above the depth of the built trees.
     auto push = [&](uint32_t r) {
         if (sp < (uint32_t)kLdsStack) lds[sp * stride] = r;
-        else if (sp < (uint32_t)(kLdsStack + kSpillStack)) spill[sp - kLdsStack] = r;
+        else if (sp < (uint32_t)(kLdsStack + kSpillStack)) stk.spill[(sp - kLdsStack) * stk.spill_stride] = r;
         else return;
         sp++;
     };
@@ -187,6 +203,7 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
     for (uint32_t trips = 0; trips < (1u << 22); trips++) {
         if ((cur & kLeafBit) == 0u) {
             const WideNode<W>& nd = nodes[cur];
+            if (COUNT) cnt->nodes++;
             float tn[W];
             uint32_t ref[W];
             const uint32_t nc = nd.nchild;
@@ -224,6 +241,7 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
         } else if (inst < 0) {
             inst = (int)leaf_start(cur);
             const InstTrav& it = S.inst_trav[inst];
+            if (COUNT) cnt->insts++;
             f3 od;
             xform_ray(it.w2o, wo, wd, o, od);
             inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
@@ -232,10 +250,11 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
             cur = it.blas_root;
             continue;
         } else {
-            const uint32_t first = leaf_start(cur), cnt = leaf_count(cur);
-            for (uint32_t k = 0; k < cnt; k++) {
+            const uint32_t first = leaf_start(cur), ntri = leaf_count(cur);
+            for (uint32_t k = 0; k < ntri; k++) {
                 const uint32_t ti = first + k;
                 float4 a = tris[3 * ti], b = tris[3 * ti + 1], c = tris[3 * ti + 2];
+                if (COUNT) cnt->tris++;
                 float t, u, v;
                 if (!tri_hit(wt, o, ld3(a), ld3(b), ld3(c), tmin, h.t, t, u, v)) continue;
                 const uint32_t prim = __float_as_uint(a.w);
@@ -260,7 +279,7 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
     pop:
         if (sp == 0) break;
         sp--;
-        cur = sp < (uint32_t)kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+        cur = sp < (uint32_t)kLdsStack ? lds[sp * stride] : stk.spill[(sp - kLdsStack) * stk.spill_stride];
     }
     return h.found;
 }

@@ -78,7 +78,12 @@ _SIGS = {
     "hh_trace_rays": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_primary_hits": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_sky_fit": [C.POINTER(abi.SkyParams), C.c_void_p, C.c_void_p],
+    "hh_context_set_profiling": [C.c_void_p, C.c_uint32],
+    "hh_context_read_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "hh_context_reset_profile": [C.c_void_p],
 }
+
+KERNEL_KINDS = ("generate", "extend", "shade", "shadow", "accumulate", "megakernel")
 
 
 def lib():
@@ -127,20 +132,40 @@ class Context:
     def sync(self):
         check(lib().hh_sync(self.h))
 
-    def set_tuning(self, bvh_width=4, extend_block=256, queue_capacity=0, persistent_blocks_per_cu=8):
-        t = abi.Tuning(bvh_width, extend_block, extend_block, queue_capacity, persistent_blocks_per_cu)
+    def set_tuning(self, bvh_width=4, extend_block=256, queue_capacity=0, persistent_blocks_per_cu=8,
+                   frames_in_flight=4):
+        t = abi.Tuning(bvh_width, extend_block, extend_block, queue_capacity, persistent_blocks_per_cu,
+                       frames_in_flight)
         check(lib().hh_context_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
         t = abi.Tuning()
         check(lib().hh_context_get_tuning(self.h, C.byref(t)))
         return dict(bvh_width=t.bvh_width, extend_block=t.extend_block, queue_capacity=t.queue_capacity,
-                    persistent_blocks_per_cu=t.persistent_blocks_per_cu)
+                    persistent_blocks_per_cu=t.persistent_blocks_per_cu, frames_in_flight=t.frames_in_flight)
 
     def stream(self):
         s = C.c_void_p()
         check(lib().hh_context_stream(self.h, C.byref(s)))
         return s.value
+
+    def set_profiling(self, timing=False, counting=False):
+        check(lib().hh_context_set_profiling(self.h, (1 if timing else 0) | (2 if counting else 0)))
+
+    def reset_profile(self):
+        check(lib().hh_context_reset_profile(self.h))
+
+    def read_profile(self):
+        """Per kernel kind: total ms and launches (HIP events on the context
+        stream), plus traversal fetch counts of the counting variant."""
+        ms = np.zeros(6, np.float64)
+        n = np.zeros(6, np.uint64)
+        c = np.zeros(6, np.uint64)
+        check(lib().hh_context_read_profile(self.h, ptr(ms), ptr(n), ptr(c)))
+        out = {k: dict(ms=float(ms[i]), launches=int(n[i])) for i, k in enumerate(KERNEL_KINDS)}
+        out["counts"] = dict(ext_nodes=int(c[0]), ext_tris=int(c[1]), ext_insts=int(c[2]), shadow_nodes=int(c[3]),
+                             shadow_tris=int(c[4]), shadow_insts=int(c[5]))
+        return out
 
     def close(self):
         if self.h:

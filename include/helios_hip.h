@@ -332,9 +332,28 @@ typedef struct hh_tuning {
     uint32_t shade_block;
     uint32_t queue_capacity;  /* rays per wavefront pass; 0 = all paths */
     uint32_t persistent_blocks_per_cu;
+    /* Progressive frames traced together by one wavefront pass when
+     * hh_integrator_render_frames is asked for several frames. Samples are
+     * still accumulated in frame order, so the image is bit-identical to
+     * rendering the frames one at a time; deep-bounce queues are F times
+     * fuller, which hides the latency tail of incoherent rays. */
+    uint32_t frames_in_flight;
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);
+
+/* ---------------------------------------------------------------------------
+ * Measurement (replaces the reference's scoped GPU timestamps,
+ * src/engine/utility/profiler.cpp:77-146). flags: bit 0 = HIP events around
+ * every kernel launch (frames are then launched directly, not from a graph);
+ * bit 1 = counting traversal variant (node / triangle / instance fetches).
+ * Kernel kinds: 0 generate, 1 extend, 2 shade, 3 shadow, 4 accumulate,
+ * 5 megakernel. counts: extension-ray nodes, triangles, instances, then the
+ * same for shadow rays.
+ * ------------------------------------------------------------------------- */
+hh_status hh_context_set_profiling(hh_context ctx, uint32_t flags);
+hh_status hh_context_read_profile(hh_context ctx, double ms[6], uint64_t launches[6], uint64_t counts[6]);
+hh_status hh_context_reset_profile(hh_context ctx);
 
 #ifdef __cplusplus
 }

@@ -206,6 +206,17 @@ def test_progressive_protocol(gpu_ctx, cornell, cornell_pair):
     assert np.array_equal(c, one)
 
 
+def test_frames_in_flight_bit_identical(gpu_ctx, small_field, field_pair):
+    """Tracing several progressive frames per wavefront pass changes nothing."""
+    g, _ = field_pair
+    imgs = []
+    for fif in (1, 3, 8):
+        gpu_ctx.set_tuning(frames_in_flight=fif)
+        imgs.append(_render_gpu(gpu_ctx, g, small_field, frames=7)[0])
+    gpu_ctx.set_tuning()
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+
+
 def test_tiled_mode_matches_untiled(gpu_ctx, oracle_mod):
     """Tiled progressive mode (128x128 tiles, max_samples each) ends at the same image."""
     from helios_amd import runtime
