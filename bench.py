@@ -39,7 +39,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=40)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=8)
     p.add_argument("--width", type=int, default=1920)
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--bounces", type=int, default=8)
@@ -51,6 +51,9 @@ def parse():
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--probe", action="store_true",
+                   help="counter-collection driver: 2 frames of the workload, then a 1 GiB copy kernel "
+                        "(the calibration point for FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -102,6 +105,14 @@ def main():
         from helios_amd.abi import HH_FLAG_MEGAKERNEL
         it.set_flags(HH_FLAG_MEGAKERNEL)
 
+    if a.probe:
+        it.render(scene, desc.camera, surf, frames=a.frames_in_flight)
+        ctx.sync()
+        gbps = ctx.bandwidth_probe(1 << 30, 1)
+        print(json.dumps({"probe": f"{a.frames_in_flight} frames (one chunk) + 1 GiB copy", "copy_gbps": gbps,
+                          "rays": it.ray_stats()}))
+        return 0
+
     it.render(scene, desc.camera, surf, frames=a.warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -143,13 +154,16 @@ def main():
     if not a.no_roofline and not a.megakernel:
         # (1) kernel durations from HIP events on the launch stream, (2) fetch
         # counts from the counting traversal variant on the same frames.
+        # Both passes trace one chunk of frames_in_flight frames, the launch
+        # shape of the timed run, so the per-launch figures describe the same
+        # dispatches rocprofv3 averages.
         ctx.set_profiling(timing=True)
         ctx.reset_profile()
-        it.render(scene, desc.camera, surf, frames=3)
+        it.render(scene, desc.camera, surf, frames=a.frames_in_flight)
         prof = ctx.read_profile()
-        ctx.set_profiling(counting=True)
+        ctx.set_profiling(timing=True, counting=True)
         ctx.reset_profile()
-        it.render(scene, desc.camera, surf, frames=3)
+        it.render(scene, desc.camera, surf, frames=a.frames_in_flight)
         cprof = ctx.read_profile()
         cnt = cprof["counts"]
         rays = it.ray_stats()["extension_rays"]
@@ -173,7 +187,10 @@ def main():
             "per_ray": {"nodes": round(cnt["ext_nodes"] / rays, 2), "tris": round(cnt["ext_tris"] / rays, 2),
                         "insts": round(cnt["ext_insts"] / rays, 2)},
         }
-        out["kernel_ms_per_frame"] = {k: round(v["ms"] / 3, 4) for k, v in prof.items() if k != "counts" and v["launches"]}
+        out["kernel_ms_per_frame"] = {k: round(v["ms"] / a.frames_in_flight, 4) for k, v in prof.items()
+                                      if k != "counts" and v["launches"]}
+
+    out["hbm_copy_gbps"] = round(ctx.bandwidth_probe(1 << 30, 4), 1)
 
     if not a.no_cpu_baseline:
         try:

@@ -429,6 +429,38 @@ hh_status hh_context_reset_profile(hh_context ctx) {
     return HH_OK;
 }
 
+hh_status hh_bandwidth_probe(hh_context ctx, uint64_t bytes, uint32_t iters, double* out_gbps) {
+    HH_ARG(ctx && out_gbps && bytes >= 16 && iters >= 1, "bad argument");
+    HH_ARG(bytes <= (uint64_t)64 << 30, "probe size above 64 GiB");
+    HH_HIP(hipSetDevice(ctx->device));
+    size_t n = bytes / 16;
+    float4 *src = nullptr, *dst = nullptr;
+    HH_HIP(hipMalloc(&src, n * 16));
+    hipError_t e = hipMalloc(&dst, n * 16);
+    if (e != hipSuccess) {
+        (void)hipFree(src);
+        return hip_fail(e, "probe allocation");
+    }
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    e = launch_fill_f4(ctx->stream, src, make_float4(1.f, 2.f, 3.f, 4.f), (uint32_t)std::min<size_t>(n, 0xFFFFFFFFu));
+    if (e == hipSuccess) e = launch_copy_probe(ctx->stream, ctx->num_cu, src, dst, n);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+    for (uint32_t i = 0; i < iters && e == hipSuccess; i++) e = launch_copy_probe(ctx->stream, ctx->num_cu, src, dst, n);
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return hip_fail(e, "bandwidth probe");
+    *out_gbps = 2.0 * (double)n * 16.0 * iters / (ms * 1e-3) / 1e9;  // read + write
+    return HH_OK;
+}
+
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t) {
     HH_ARG(ctx && t, "null argument");
     t->bvh_width = (uint32_t)ctx->tune.bvh_width;

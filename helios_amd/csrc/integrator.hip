@@ -674,6 +674,17 @@ __global__ void k_fill_f4(float4* p, float4 v, uint32_t n) {
     if (i < n) p[i] = v;
 }
 
+// Plain streaming copy of known size: the reference point for HBM byte
+// counters (FETCH_SIZE / WRITE_SIZE) and for the achievable copy bandwidth.
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(hipStream_t st, int num_cu, const float4* src, float4* dst, size_t n) {
+    k_copy_probe<<<num_cu * 16, 256, 0, st>>>(src, dst, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------

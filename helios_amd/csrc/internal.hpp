@@ -123,5 +123,6 @@ hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const f
 // Host Hosek-Wilkie fit (sky_host.cpp)
 void sky_coefficients(float turbidity, float albedo, float normalized_sun_y, const float sun_dir[3], float out[30]);
 hipError_t launch_fill_f4(hipStream_t st, float4* p, float4 v, uint32_t n);
+hipError_t launch_copy_probe(hipStream_t st, int num_cu, const float4* src, float4* dst, size_t n);
 
 }  // namespace hh

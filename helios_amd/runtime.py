@@ -81,6 +81,7 @@ _SIGS = {
     "hh_context_set_profiling": [C.c_void_p, C.c_uint32],
     "hh_context_read_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "hh_context_reset_profile": [C.c_void_p],
+    "hh_bandwidth_probe": [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)],
 }
 
 KERNEL_KINDS = ("generate", "extend", "shade", "shadow", "accumulate", "megakernel")
@@ -154,6 +155,12 @@ class Context:
 
     def reset_profile(self):
         check(lib().hh_context_reset_profile(self.h))
+
+    def bandwidth_probe(self, nbytes=1 << 30, iters=4):
+        """Read+write GB/s of a plain device copy of nbytes (HBM calibration)."""
+        out = C.c_double()
+        check(lib().hh_bandwidth_probe(self.h, nbytes, iters, C.byref(out)))
+        return out.value
 
     def read_profile(self):
         """Per kernel kind: total ms and launches (HIP events on the context

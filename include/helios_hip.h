@@ -354,6 +354,9 @@ hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);
 hh_status hh_context_set_profiling(hh_context ctx, uint32_t flags);
 hh_status hh_context_read_profile(hh_context ctx, double ms[6], uint64_t launches[6], uint64_t counts[6]);
 hh_status hh_context_reset_profile(hh_context ctx);
+/* Streams `bytes` through a plain copy kernel `iters` times and returns the
+ * achieved read+write GB/s: the calibration point for HBM byte counters. */
+hh_status hh_bandwidth_probe(hh_context ctx, uint64_t bytes, uint32_t iters, double* out_gbps);
 
 #ifdef __cplusplus
 }
