@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--bounces", type=int, default=8)
     p.add_argument("--bvh-width", type=int, default=4)
+    p.add_argument("--bvh-compressed", type=int, default=0)
     p.add_argument("--block", type=int, default=64)
     p.add_argument("--blocks-per-cu", type=int, default=32)
     p.add_argument("--frames-in-flight", type=int, default=8)
@@ -93,7 +94,8 @@ def main():
     desc = scenegen.icosphere_field(seed=0, width=a.width, height=a.height, max_ray_bounces=a.bounces)
     ctx = runtime.Context(0)
     ctx.set_tuning(bvh_width=a.bvh_width, extend_block=a.block, persistent_blocks_per_cu=a.blocks_per_cu,
-                   frames_in_flight=a.frames_in_flight)
+                   frames_in_flight=a.frames_in_flight, bvh_compressed=a.bvh_compressed)
+    node_bytes = 64 if (a.bvh_compressed and a.bvh_width == 4) else NODE_BYTES[a.bvh_width]
     t0 = time.perf_counter()
     scene = runtime.Scene(ctx, desc)
     build_s = time.perf_counter() - t0
@@ -141,7 +143,8 @@ def main():
             "workload": "1920x1080 icosphere field: 64 level-5 displaced icospheres (1,310,720 tris) + ground + "
                         "2,000 alpha leaf quads, Hosek-Wilkie sky + sun + 4 point lights, 8 bounces, 1 spp/frame",
             "width": a.width, "height": a.height, "triangles": desc.num_triangles, "bounces": a.bounces,
-            "spp_per_frame": 1, "frames_in_flight": a.frames_in_flight, "bvh_width": a.bvh_width, "block": a.block, "blocks_per_cu": a.blocks_per_cu,
+            "spp_per_frame": 1, "frames_in_flight": a.frames_in_flight, "bvh_width": a.bvh_width,
+            "bvh_compressed": bool(a.bvh_compressed), "block": a.block, "blocks_per_cu": a.blocks_per_cu,
             "integrator": "megakernel" if a.megakernel else "wavefront+hipGraph",
         },
         "gsample_bounces_per_s": round(gsb, 4),
@@ -169,7 +172,7 @@ def main():
         rays = it.ray_stats()["extension_rays"]
         ctx.set_profiling()
         ext = prof["extend"]
-        alg_bytes = (cnt["ext_nodes"] * NODE_BYTES[a.bvh_width] + cnt["ext_tris"] * TRI_BYTES +
+        alg_bytes = (cnt["ext_nodes"] * node_bytes + cnt["ext_tris"] * TRI_BYTES +
                      cnt["ext_insts"] * INST_BYTES + rays * RAY_QUEUE_BYTES)
         per_launch = alg_bytes / max(cprof["extend"]["launches"], 1)
         avg_ms = ext["ms"] / max(ext["launches"], 1)

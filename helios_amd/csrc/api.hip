@@ -372,6 +372,7 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->persistent_blocks_per_cu >= 1 && t->persistent_blocks_per_cu <= 64, "blocks per CU must be 1..64");
     HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 64, "frames_in_flight must be 1..64");
     ctx->tune.frames_in_flight = t->frames_in_flight;
+    ctx->tune.bvh_compressed = t->bvh_compressed != 0 && t->bvh_width == 4;
     ctx->tune.bvh_width = (int)t->bvh_width;
     ctx->tune.extend_block = (int)t->extend_block;
     ctx->tune.shade_block = (int)t->extend_block;
@@ -469,6 +470,7 @@ hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t) {
     t->queue_capacity = ctx->tune.queue_capacity;
     t->persistent_blocks_per_cu = (uint32_t)ctx->tune.blocks_per_cu;
     t->frames_in_flight = ctx->tune.frames_in_flight;
+    t->bvh_compressed = ctx->tune.bvh_compressed ? 1u : 0u;
     return HH_OK;
 }
 
@@ -875,6 +877,17 @@ hh_status hh_scene_commit(hh_scene s) {
     }
     s->bvh_stats[2] = tlas_nodes;
     s->bvh_stats[3] = global_meshes.size();
+    // Optional 64-B quantized BVH4 nodes (same topology and indices).
+    int layout = W;
+    if (ctx->tune.bvh_compressed && W == 4) {
+        void* q = nullptr;
+        const uint32_t total_nodes = node_base + tlas_nodes;
+        HH_HIP(dalloc(s, size_t(total_nodes) * sizeof(QNode4), &q));
+        HH_HIP(quantize_bvh4(ctx->stream, d_nodes, q, total_nodes));
+        HH_HIP(hipStreamSynchronize(ctx->stream));
+        d_nodes = q;
+        layout = kLayoutQ4;
+    }
     // ---- textures --------------------------------------------------------------
     std::vector<TexDesc> tdesc;
     std::vector<float4> texels;
@@ -915,7 +928,7 @@ hh_status hh_scene_commit(hh_scene s) {
     g.env_size = env_size;
     g.num_lights = num_lights;
     g.tlas_root = tlas_root;
-    g.bvh_width = (uint32_t)W;
+    g.bvh_width = (uint32_t)layout;
     HH_HIP(upload(s, trav, (InstTrav**)&g.inst_trav));
     HH_HIP(upload(s, inst_data, (InstanceData**)&g.inst));
     HH_HIP(upload(s, mat_data, (MaterialData**)&g.mats));
@@ -1318,7 +1331,6 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
             scenes.push_back(views[i].scene->gpu);
             key += "|" + std::to_string(views[i].scene->generation);
         }
-    HH_ARG(scenes[0].bvh_width == (uint32_t)ctx->tune.bvh_width, "scene BVH width differs from the context tuning");
     for (auto& sg : scenes) HH_ARG(sg.bvh_width == scenes[0].bvh_width, "scenes of one batch must share the BVH width");
     unsigned long long zero[2] = {0, 0};
     uint32_t total_px = 0;

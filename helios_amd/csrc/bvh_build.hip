@@ -286,6 +286,68 @@ __global__ void k_gather_tris(const float4* __restrict__ src, const uint32_t* __
     dst[3 * i + 2] = src[3 * p + 2];
 }
 
+// Quantization of one axis of one child box: the decoded planes
+// fmaf(q, 2^e, origin) must satisfy lo' <= lo and hi' >= hi exactly in float.
+__device__ inline bool quant_axis(float o, float s, float clo, float chi, uint8_t& qlo, uint8_t& qhi) {
+    int a = (int)floor(((double)clo - (double)o) / (double)s);
+    a = min(max(a, 0), 255);
+    while (a > 0 && fmaf((float)a, s, o) > clo) a--;
+    int b = (int)ceil(((double)chi - (double)o) / (double)s);
+    b = min(max(b, 0), 255);
+    while (b < 255 && fmaf((float)b, s, o) < chi) b++;
+    qlo = (uint8_t)a;
+    qhi = (uint8_t)b;
+    return fmaf((float)a, s, o) <= clo && fmaf((float)b, s, o) >= chi;
+}
+
+__device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
+
+// BVH4 (float boxes, 128 B) -> quantized BVH4 (64 B), same topology.
+__global__ void k_quantize4(const WideNode<4>* __restrict__ src, QNode4* __restrict__ dst, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    WideNode<4> s = src[i];
+    QNode4 q;
+    q.nchild = (uint8_t)s.nchild;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const float* L[3] = {s.lox, s.loy, s.loz};
+    const float* H[3] = {s.hix, s.hiy, s.hiz};
+    for (uint32_t k = 0; k < s.nchild; k++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], L[a][k]);
+            hi[a] = fmaxf(hi[a], H[a][k]);
+        }
+    if (s.nchild == 0)
+        for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0.f;
+    uint8_t* QL[3] = {q.qlox, q.qloy, q.qloz};
+    uint8_t* QH[3] = {q.qhix, q.qhiy, q.qhiz};
+    int8_t* E[3] = {&q.ex, &q.ey, &q.ez};
+    float* O[3] = {&q.ox, &q.oy, &q.oz};
+    for (int a = 0; a < 3; a++) {
+        *O[a] = lo[a];
+        double ext = (double)hi[a] - (double)lo[a];
+        int e = -126;
+        if (ext > 0.0) {
+            int fe;
+            frexp(ext / 255.0, &fe);
+            e = max(fe, -126);
+        }
+        for (;; e++) {
+            bool ok = true;
+            float sc = exp2i(e);
+            for (int k = 0; k < 4; k++) {
+                if ((uint32_t)k < s.nchild) ok &= quant_axis(lo[a], sc, L[a][k], H[a][k], QL[a][k], QH[a][k]);
+                else QL[a][k] = QH[a][k] = 0;
+            }
+            if (ok || e >= 127) break;
+        }
+        *E[a] = (int8_t)e;
+    }
+    for (int k = 0; k < 4; k++) q.child[k] = s.child[k];
+    q.pad[0] = q.pad[1] = 0;
+    dst[i] = q;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -403,6 +465,12 @@ hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, cons
                           tri_base, leaf_size, 0, order, sc, num_nodes, root_box);
     HH_TRY(e);
     k_gather_tris<<<blocks(n, B), B, 0, st>>>(d_tri_src, order, n, d_tri_pool + 3 * (size_t)tri_base);
+    return hipGetLastError();
+}
+
+hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    k_quantize4<<<blocks(n, 128), 128, 0, st>>>((const WideNode<4>*)d_src, (QNode4*)d_dst, n);
     return hipGetLastError();
 }
 

@@ -53,6 +53,27 @@ struct alignas(16) WideNode {
 static_assert(sizeof(WideNode<4>) == 128, "BVH4 node must be one 128-B line");
 static_assert(sizeof(WideNode<8>) == 256, "BVH8 node must be two 128-B lines");
 
+// Quantized BVH4 node, 64 B (four 16-B loads instead of eight): child boxes
+// as 8-bit offsets from the node origin in units of 2^e per axis. The decoded
+// plane fmaf(q, 2^e, origin) is rounded outward at build time, so a decoded
+// box always contains the exact child box and the robust slab test stays
+// conservative.
+struct alignas(16) QNode4 {
+    float ox, oy, oz;
+    int8_t ex, ey, ez;
+    uint8_t nchild;
+    uint8_t qlox[4], qhix[4], qloy[4], qhiy[4], qloz[4], qhiz[4];
+    uint32_t child[4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(QNode4) == 64, "quantized BVH4 node is 64 B");
+
+// Node layouts a scene's BVH can use (SceneGPU::bvh_width holds one of these).
+constexpr int kLayoutB4 = 4;    // WideNode<4>, float boxes, 128 B
+constexpr int kLayoutB8 = 8;    // WideNode<8>, float boxes, 256 B
+constexpr int kLayoutQ4 = 16;   // QNode4, quantized boxes, 64 B
+__host__ __device__ constexpr int layout_width(int layout) { return layout == kLayoutB8 ? 8 : 4; }
+
 struct alignas(16) InstTrav {
     float w2o[12];        // row-major 3x4 world->object
     uint32_t blas_root;   // global node index

@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(BS) k_extend(const SceneGPU* __restrict__ scen
                                                unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
                                                uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
-    const LaneStack stk{&stack[threadIdx.x], BS, spill + blockIdx.x * BS + threadIdx.x, gridDim.x * BS};
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     const uint32_t n = *count_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)n);
     const uint32_t stride = gridDim.x * BS;
@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(BS) k_shadow(const SceneGPU* __restrict__ scen
                                                unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
                                                uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
-    const LaneStack stk{&stack[threadIdx.x], BS, spill + blockIdx.x * BS + threadIdx.x, gridDim.x * BS};
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     const uint32_t n = *count_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], (unsigned long long)n);
     const uint32_t stride = gridDim.x * BS;
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
                                                    uint32_t nv, uint32_t total, float4* __restrict__ Lout,
                                                    unsigned long long* __restrict__ stats, uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
-    const LaneStack stk{&stack[threadIdx.x], BS, spill + blockIdx.x * BS + threadIdx.x, gridDim.x * BS};
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     const uint32_t stride = gridDim.x * BS;
     unsigned long long n_ext = 0, n_sh = 0;
     for (uint32_t slot = blockIdx.x * BS + threadIdx.x; slot < total; slot += stride) {
@@ -574,7 +574,7 @@ __global__ void __launch_bounds__(256) k_trace_rays(const SceneGPU* __restrict__
                                                     uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * 256];
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const LaneStack stk{&stack[threadIdx.x], 256, spill + i, gridDim.x * 256};
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), 256, to_global(spill + i), gridDim.x * 256};
     if (i >= n) return;
     const SceneGPU& S = *scene;
     float4 o = rays[2 * i], d = rays[2 * i + 1];
@@ -599,7 +599,7 @@ __global__ void __launch_bounds__(256) k_primary(const SceneGPU* __restrict__ sc
                                                  uint32_t* __restrict__ out_ids, uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * 256];
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const LaneStack stk{&stack[threadIdx.x], 256, spill + i, gridDim.x * 256};
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), 256, to_global(spill + i), gridDim.x * 256};
     if (i >= w * h_) return;
     uint32_t x = i % w, y = i / w;
     const SceneGPU& S = *scene;
@@ -765,37 +765,44 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
 hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const SceneGPU* h_scenes, const ViewGPU* d_views,
                         const ViewGPU* h_views, uint32_t nv, uint32_t total, uint32_t max_bounces, Wavefront& wf,
                         bool mega) {
-    const int W = h_scenes[0].bvh_width;
+    const int L = h_scenes[0].bvh_width;  // node layout id
     const int bs = cfg.tune.extend_block;
     if (cfg.prof && cfg.prof->count && cfg.prof->d_counts) {
-        if (W == 8) return frame_impl<8, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
-        return frame_impl<4, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
+        if (L == kLayoutB8) return frame_impl<kLayoutB8, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
+        if (L == kLayoutQ4) return frame_impl<kLayoutQ4, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
+        return frame_impl<kLayoutB4, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
     }
-#define HH_DISPATCH(WW, BB) return frame_impl<WW, BB, false>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
-    if (W == 8) {
-        if (bs == 64) HH_DISPATCH(8, 64);
-        if (bs == 128) HH_DISPATCH(8, 128);
-        HH_DISPATCH(8, 256);
-    }
-    if (bs == 64) HH_DISPATCH(4, 64);
-    if (bs == 128) HH_DISPATCH(4, 128);
-    HH_DISPATCH(4, 256);
+#define HH_DISPATCH(LL, BB) return frame_impl<LL, BB, false>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
+#define HH_BLOCKS(LL)                      \
+    do {                                   \
+        if (bs == 64) HH_DISPATCH(LL, 64);   \
+        if (bs == 128) HH_DISPATCH(LL, 128); \
+        HH_DISPATCH(LL, 256);                \
+    } while (0)
+    if (L == kLayoutB8) HH_BLOCKS(kLayoutB8);
+    if (L == kLayoutQ4) HH_BLOCKS(kLayoutQ4);
+    HH_BLOCKS(kLayoutB4);
+#undef HH_BLOCKS
 #undef HH_DISPATCH
 }
 
 hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,
                              uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_ids, uint32_t* spill) {
     if (n == 0) return hipSuccess;
-    if (h_scene.bvh_width == 8) k_trace_rays<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
-    else k_trace_rays<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
+    const int L = h_scene.bvh_width;
+    if (L == kLayoutB8) k_trace_rays<kLayoutB8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
+    else if (L == kLayoutQ4) k_trace_rays<kLayoutQ4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
+    else k_trace_rays<kLayoutB4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_rays, n, flags, d_hit, d_ids, spill);
     return hipGetLastError();
 }
 
 hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
                                uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_ids, uint32_t* spill) {
     uint32_t n = w * h;
-    if (h_scene.bvh_width == 8) k_primary<8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
-    else k_primary<4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
+    const int L = h_scene.bvh_width;
+    if (L == kLayoutB8) k_primary<kLayoutB8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
+    else if (L == kLayoutQ4) k_primary<kLayoutQ4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
+    else k_primary<kLayoutB4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, frame, d_hit, d_ids, spill);
     return hipGetLastError();
 }
 

@@ -35,6 +35,7 @@ hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, cons
                       float root_box[6]);
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
                       uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes);
+hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_t n);
 
 // Wavefront state of one render call (device buffers sized for max_paths).
 struct Wavefront {
@@ -75,6 +76,8 @@ struct Tuning {
     int blocks_per_cu = 8;
     int leaf_size = 4;
     uint32_t frames_in_flight = 4;
+    bool bvh_compressed = false;
+    int layout() const { return bvh_compressed && bvh_width == 4 ? kLayoutQ4 : bvh_width; }
 };
 
 // Optional per-kernel profiling of a frame: HIP events around every launch

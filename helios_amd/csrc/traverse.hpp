@@ -25,16 +25,30 @@ namespace hh {
 constexpr int kLdsStack = 24;
 constexpr int kSpillStack = kSpillStackDepth;
 
+// Explicit address spaces: pointers that arrive through structs are generic,
+// and generic (flat) accesses both serialize on two wait counters and, for
+// the LDS stack, bypass the ds_* path. Traversal casts to these once. The
+// host pass only parses device bodies, so there the qualifiers are empty.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HH_GLOBAL __attribute__((address_space(1)))
+#define HH_LDS __attribute__((address_space(3)))
+#else
+#define HH_GLOBAL
+#define HH_LDS
+#endif
+
 // Where one lane keeps its traversal stack: the first kLdsStack entries in LDS
 // (entry k at lds[k * lds_stride]), deeper entries in a global per-thread
 // spill area (entry k at spill[(k - kLdsStack) * spill_stride]). Keeping the
 // spill out of private memory keeps the kernels free of scratch.
 struct LaneStack {
-    uint32_t* lds;
+    HH_LDS uint32_t* lds;
     uint32_t lds_stride;
-    uint32_t* spill;
+    HH_GLOBAL uint32_t* spill;
     uint32_t spill_stride;
 };
+__device__ inline HH_LDS uint32_t* to_lds(uint32_t* p) { return (HH_LDS uint32_t*)p; }
+__device__ inline HH_GLOBAL uint32_t* to_global(uint32_t* p) { return (HH_GLOBAL uint32_t*)p; }
 
 // Per-ray fetch counts of the counting variant (algorithmic bytes).
 struct TravCount {
@@ -106,7 +120,8 @@ __device__ inline bool tri_hit(const WT& pr, f3 o, f3 v0, f3 v1, f3 v2, float tm
 }
 
 // World -> object (row-major 3x4), fixed operation order.
-__device__ inline void xform_ray(const float* m, f3 o, f3 d, f3& oo, f3& od) {
+template <typename P>
+__device__ inline void xform_ray(P m, f3 o, f3 d, f3& oo, f3& od) {
     oo.x = ((m[0] * o.x + m[1] * o.y) + m[2] * o.z) + m[3];
     oo.y = ((m[4] * o.x + m[5] * o.y) + m[6] * o.z) + m[7];
     oo.z = ((m[8] * o.x + m[9] * o.y) + m[10] * o.z) + m[11];
@@ -159,6 +174,57 @@ __device__ inline float hit_alpha(const SceneGPU& S, uint32_t inst, uint32_t geo
     return sample_tex(S, m.tex0[0], tu, tv).w;
 }
 
+// One slab test in the robust form (Ize 2013): returns the entry distance or
+// +inf on a miss. min/max drop a NaN operand (0 * inf for zero directions).
+__device__ inline float slab(float lox, float hix, float loy, float hiy, float loz, float hiz, f3 o, f3 inv, float tmin,
+                             float tmax) {
+    float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
+    float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
+    float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
+    float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    return a <= b * 1.0000004f ? a : INFINITY;
+}
+
+__device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
+__device__ inline float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 255u); }
+
+// Tests the children of node `cur`: tn[k] = entry distance (+inf = miss or
+// absent child), ref[k] = child reference.
+template <int L>
+__device__ inline void node_test(const void* nodes, uint32_t cur, f3 o, f3 inv, float tmin, float tmax, float* tn,
+                                 uint32_t* ref) {
+    if constexpr (L == kLayoutQ4) {
+        const HH_GLOBAL uint4* p = (const HH_GLOBAL uint4*)nodes + 4 * (size_t)cur;
+        const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+        const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+        const float sx = exp2i((int)(int8_t)(w0.w & 255u)), sy = exp2i((int)(int8_t)((w0.w >> 8) & 255u)),
+                    sz = exp2i((int)(int8_t)((w0.w >> 16) & 255u));
+        const uint32_t nc = w0.w >> 24;
+        ref[0] = w2.z;
+        ref[1] = w2.w;
+        ref[2] = w3.x;
+        ref[3] = w3.y;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float t = slab(fmaf(qbyte(w1.x, k), sx, ox), fmaf(qbyte(w1.y, k), sx, ox), fmaf(qbyte(w1.z, k), sy, oy),
+                           fmaf(qbyte(w1.w, k), sy, oy), fmaf(qbyte(w2.x, k), sz, oz), fmaf(qbyte(w2.y, k), sz, oz), o, inv,
+                           tmin, tmax);
+            tn[k] = (uint32_t)k < nc ? t : INFINITY;
+        }
+    } else {
+        constexpr int W = layout_width(L);
+        const HH_GLOBAL WideNode<W>& nd = ((const HH_GLOBAL WideNode<W>*)nodes)[cur];
+        const uint32_t nc = nd.nchild;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            float t = slab(nd.lox[k], nd.hix[k], nd.loy[k], nd.hiy[k], nd.loz[k], nd.hiz[k], o, inv, tmin, tmax);
+            tn[k] = (uint32_t)k < nc ? t : INFINITY;
+            ref[k] = nd.child[k];
+        }
+    }
+}
+
 __device__ inline bool key_less(uint32_t i1, uint32_t g1, uint32_t p1, uint32_t i2, uint32_t g2, uint32_t p2) {
     if (i1 != i2) return i1 < i2;
     if (g1 != g2) return g1 < g2;
@@ -170,12 +236,12 @@ __device__ inline bool key_less(uint32_t i1, uint32_t g1, uint32_t p1, uint32_t 
 // rflags: HH_TRACE_OPAQUE (bit 0) disables the any-hit alpha test.
 // `stk` says where this lane's stack lives.
 // ---------------------------------------------------------------------------
-template <int W, bool ANY, bool COUNT = false>
+template <int L, bool ANY, bool COUNT = false>
 __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, float tmax, uint32_t rflags, HitRec& h,
                                  const LaneStack& stk, TravCount* cnt = nullptr) {
-    const WideNode<W>* nodes = (const WideNode<W>*)S.nodes;
-    const float4* tris = S.tris;
-    uint32_t* const lds = stk.lds;
+    constexpr int W = layout_width(L);
+    const HH_GLOBAL float4* tris = (const HH_GLOBAL float4*)S.tris;
+    HH_LDS uint32_t* const lds = stk.lds;
     const uint32_t stride = stk.lds_stride;
     f3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
     f3 o = wo, inv = winv;
@@ -202,22 +268,10 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
     // malformed tree contained a cycle.
     for (uint32_t trips = 0; trips < (1u << 22); trips++) {
         if ((cur & kLeafBit) == 0u) {
-            const WideNode<W>& nd = nodes[cur];
             if (COUNT) cnt->nodes++;
             float tn[W];
             uint32_t ref[W];
-            const uint32_t nc = nd.nchild;
-#pragma unroll
-            for (int k = 0; k < W; k++) {
-                float t0x = (nd.lox[k] - o.x) * inv.x, t1x = (nd.hix[k] - o.x) * inv.x;
-                float t0y = (nd.loy[k] - o.y) * inv.y, t1y = (nd.hiy[k] - o.y) * inv.y;
-                float t0z = (nd.loz[k] - o.z) * inv.z, t1z = (nd.hiz[k] - o.z) * inv.z;
-                float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-                float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), h.t));
-                bool hit = (uint32_t)k < nc && a <= b * 1.0000004f;
-                tn[k] = hit ? a : INFINITY;
-                ref[k] = nd.child[k];
-            }
+            node_test<L>(S.nodes, cur, o, inv, tmin, h.t, tn, ref);
             // sort ascending by entry distance (small network, registers only)
 #pragma unroll
             for (int i = 1; i < W; i++)
@@ -240,7 +294,7 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
             goto pop;
         } else if (inst < 0) {
             inst = (int)leaf_start(cur);
-            const InstTrav& it = S.inst_trav[inst];
+            const HH_GLOBAL InstTrav& it = ((const HH_GLOBAL InstTrav*)S.inst_trav)[inst];
             if (COUNT) cnt->insts++;
             f3 od;
             xform_ray(it.w2o, wo, wd, o, od);

@@ -338,6 +338,9 @@ typedef struct hh_tuning {
      * rendering the frames one at a time; deep-bounce queues are F times
      * fuller, which hides the latency tail of incoherent rays. */
     uint32_t frames_in_flight;
+    /* 1 = quantized 64-B BVH4 nodes (8-bit child boxes, outward rounded);
+     * applies to bvh_width 4 and takes effect at the next scene commit. */
+    uint32_t bvh_compressed;
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);

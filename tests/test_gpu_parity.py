@@ -206,6 +206,29 @@ def test_progressive_protocol(gpu_ctx, cornell, cornell_pair):
     assert np.array_equal(c, one)
 
 
+@pytest.mark.parametrize("layout", [dict(bvh_width=8), dict(bvh_width=4, bvh_compressed=True)])
+def test_bvh_layouts_bit_identical(gpu_ctx, small_field, field_pair, oracle_mod, layout):
+    """BVH8 and the quantized BVH4 return the same hits as the oracle and the
+    same images as the float BVH4 (traversal results are order independent)."""
+    from helios_amd import runtime
+    g4, o = field_pair
+    ref_img, _ = _render_gpu(gpu_ctx, g4, small_field, frames=3)
+    gpu_ctx.set_tuning(**layout)
+    try:
+        g = runtime.Scene(gpu_ctx, small_field)
+        rays = scenegen.random_rays(30000, (-90, 0.5, -90), (90, 25, 90), seed=8, tmin=1e-4)
+        for flags in (0, HH_TRACE_OPAQUE):
+            hg = g.trace_rays(rays, flags)
+            ho = o.trace_rays(rays, flags)
+            assert not id_mismatch(hg, ho).any()
+            assert np.array_equal(hg["t"], ho["t"])
+        img, _ = _render_gpu(gpu_ctx, g, small_field, frames=3)
+        assert np.array_equal(img, ref_img)
+        g.close()
+    finally:
+        gpu_ctx.set_tuning()
+
+
 def test_frames_in_flight_bit_identical(gpu_ctx, small_field, field_pair):
     """Tracing several progressive frames per wavefront pass changes nothing."""
     g, _ = field_pair
