@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--block", type=int, default=64)
     p.add_argument("--blocks-per-cu", type=int, default=32)
     p.add_argument("--frames-in-flight", type=int, default=8)
+    p.add_argument("--refill-lanes", type=int, default=32)
+    p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
     p.add_argument("--megakernel", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -94,7 +96,8 @@ def main():
     desc = scenegen.icosphere_field(seed=0, width=a.width, height=a.height, max_ray_bounces=a.bounces)
     ctx = runtime.Context(0)
     ctx.set_tuning(bvh_width=a.bvh_width, extend_block=a.block, persistent_blocks_per_cu=a.blocks_per_cu,
-                   frames_in_flight=a.frames_in_flight, bvh_compressed=a.bvh_compressed)
+                   frames_in_flight=a.frames_in_flight, bvh_compressed=a.bvh_compressed,
+                   refill_lanes=a.refill_lanes, traversal=a.traversal)
     node_bytes = 64 if (a.bvh_compressed and a.bvh_width == 4) else NODE_BYTES[a.bvh_width]
     t0 = time.perf_counter()
     scene = runtime.Scene(ctx, desc)
@@ -145,6 +148,7 @@ def main():
             "width": a.width, "height": a.height, "triangles": desc.num_triangles, "bounces": a.bounces,
             "spp_per_frame": 1, "frames_in_flight": a.frames_in_flight, "bvh_width": a.bvh_width,
             "bvh_compressed": bool(a.bvh_compressed), "block": a.block, "blocks_per_cu": a.blocks_per_cu,
+            "refill_lanes": a.refill_lanes, "traversal": "if-if" if a.traversal else "while-while",
             "integrator": "megakernel" if a.megakernel else "wavefront+hipGraph",
         },
         "gsample_bounces_per_s": round(gsb, 4),
@@ -169,7 +173,8 @@ def main():
         it.render(scene, desc.camera, surf, frames=a.frames_in_flight)
         cprof = ctx.read_profile()
         cnt = cprof["counts"]
-        rays = it.ray_stats()["extension_rays"]
+        rstats = it.ray_stats()
+        rays, srays = rstats["extension_rays"], max(rstats["shadow_rays"], 1)
         ctx.set_profiling()
         ext = prof["extend"]
         alg_bytes = (cnt["ext_nodes"] * node_bytes + cnt["ext_tris"] * TRI_BYTES +
@@ -189,6 +194,9 @@ def main():
             "avg_launch_ms": round(avg_ms, 4),
             "per_ray": {"nodes": round(cnt["ext_nodes"] / rays, 2), "tris": round(cnt["ext_tris"] / rays, 2),
                         "insts": round(cnt["ext_insts"] / rays, 2)},
+            "shadow_per_ray": {"nodes": round(cnt["shadow_nodes"] / srays, 2),
+                               "tris": round(cnt["shadow_tris"] / srays, 2),
+                               "insts": round(cnt["shadow_insts"] / srays, 2)},
         }
         out["kernel_ms_per_frame"] = {k: round(v["ms"] / a.frames_in_flight, 4) for k, v in prof.items()
                                       if k != "counts" and v["launches"]}

@@ -371,6 +371,10 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->extend_block == 64 || t->extend_block == 128 || t->extend_block == 256, "extend_block must be 64/128/256");
     HH_ARG(t->persistent_blocks_per_cu >= 1 && t->persistent_blocks_per_cu <= 64, "blocks per CU must be 1..64");
     HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 64, "frames_in_flight must be 1..64");
+    HH_ARG(t->refill_lanes >= 1 && t->refill_lanes <= 64, "refill_lanes must be 1..64");
+    HH_ARG(t->traversal <= 1, "traversal must be 0 (while-while) or 1 (if-if)");
+    ctx->tune.refill_lanes = t->refill_lanes;
+    ctx->tune.traversal = (int)t->traversal;
     ctx->tune.frames_in_flight = t->frames_in_flight;
     ctx->tune.bvh_compressed = t->bvh_compressed != 0 && t->bvh_width == 4;
     ctx->tune.bvh_width = (int)t->bvh_width;
@@ -471,6 +475,8 @@ hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t) {
     t->persistent_blocks_per_cu = (uint32_t)ctx->tune.blocks_per_cu;
     t->frames_in_flight = ctx->tune.frames_in_flight;
     t->bvh_compressed = ctx->tune.bvh_compressed ? 1u : 0u;
+    t->refill_lanes = ctx->tune.refill_lanes;
+    t->traversal = (uint32_t)ctx->tune.traversal;
     return HH_OK;
 }
 

@@ -319,6 +319,8 @@ __global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ vie
         counters[0] = total;
         counters[1] = 0;
         counters[2] = 0;
+        counters[4] = 0;
+        counters[5] = 0;
     }
 }
 
@@ -337,28 +339,72 @@ __device__ inline void flush_counts(const TravCount& tc, unsigned long long* out
     }
 }
 
-template <int W, int BS, bool COUNT>
-__global__ void __launch_bounds__(BS) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+// Persistent queue traversal: every wave keeps up to 64 rays in flight and,
+// whenever at least `refill` of its lanes have finished (or all have), claims
+// the next rays of the queue for them with one atomic on `head`. Lanes step
+// their rays independently, so a wave is not held by its longest ray.
+// (Claiming larger private chunks per atomic was measured slower: it starves
+// waves of work when the deep-bounce queues are short.)
+struct QueueCursor {
+    uint32_t n, refill;
+    uint32_t* head;
+    bool drained;  // the queue is exhausted
+    __device__ inline QueueCursor(uint32_t n_, uint32_t refill_, uint32_t* head_)
+        : n(n_), refill(refill_), head(head_), drained(false) {}
+    // Queue index for this lane (>= n: none). Called by the whole wave in
+    // converged control flow, so lane 0 is active for the atomic.
+    __device__ inline uint32_t next(bool idle_lane) {
+        if (drained) return 0xFFFFFFFFu;
+        const unsigned long long idle = __ballot(idle_lane);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle < refill && nidle != 64u) return 0xFFFFFFFFu;
+        uint32_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(head, nidle);
+        base = __builtin_amdgcn_readfirstlane(base);
+        drained = base + nidle >= n;
+        const uint32_t lane = __lane_id();
+        const unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
+        return idle_lane ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
+    }
+};
+
+// Six waves per SIMD: the register budget (<= 80 VGPRs) that lets the LDS
+// stacks of 24 waves share one CU; measured faster than five waves with no
+// spills at all.
+template <int LY, int BS, int MODE, bool COUNT>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                                const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count_p,
-                                               uint32_t rflags, float4* __restrict__ hit, uint32_t* __restrict__ hit_tri,
+                                               uint32_t* __restrict__ head, uint32_t refill, uint32_t rflags,
+                                               float4* __restrict__ hit, uint32_t* __restrict__ hit_tri,
                                                unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
                                                uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     const uint32_t n = *count_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)n);
-    const uint32_t stride = gridDim.x * BS;
+    QueueCursor qc(n, refill, head);
     TravCount tc{0u, 0u, 0u};
-    for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n; q += stride) {
-        uint32_t slot = q_slot[q];
-        uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
-        const SceneGPU& S = scenes[views[view].scene];
-        float4 o = q_o[q], d = q_d[q];
-        HitRec h;
-        trace_ray<W, false, COUNT>(S, ld3(o), ld3(d), o.w, d.w, rflags, h, stk, &tc);
-        hit[q] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.inst));
-        hit_tri[q] = h.found ? h.tri : 0xFFFFFFFFu;
+    Tracer<LY, false, COUNT> tr;
+    uint32_t q = 0;
+    bool active = false;
+    for (;;) {
+        const uint32_t nq = qc.next(!active);
+        if (nq < n) {
+            const uint32_t slot = q_slot[nq];
+            const uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
+            const float4 o = q_o[nq], d = q_d[nq];
+            tr.begin(scenes[views[view].scene], ld3(o), ld3(d), o.w, d.w, rflags, &tc);
+            q = nq;
+            active = true;
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active && tr.template step<MODE>(stk)) {
+            const HitRec& h = tr.h;
+            hit[q] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.inst));
+            hit_tri[q] = h.found ? h.tri : 0xFFFFFFFFu;
+            active = false;
+        }
     }
     if (COUNT) flush_counts(tc, tcounts);
 }
@@ -444,38 +490,54 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
     }
 }
 
-template <int W, int BS, bool COUNT>
-__global__ void __launch_bounds__(BS) k_shadow(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+template <int LY, int BS, int MODE, bool COUNT>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_shadow(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ s_o, const float4* __restrict__ s_d,
                                                const float4* __restrict__ s_c, const uint32_t* __restrict__ s_slot,
-                                               const uint32_t* __restrict__ count_p, float4* __restrict__ L,
+                                               const uint32_t* __restrict__ count_p, uint32_t* __restrict__ head,
+                                               uint32_t refill, float4* __restrict__ L,
                                                unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
                                                uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     const uint32_t n = *count_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], (unsigned long long)n);
-    const uint32_t stride = gridDim.x * BS;
+    QueueCursor qc(n, refill, head);
     TravCount tc{0u, 0u, 0u};
-    for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n; q += stride) {
-        uint32_t slot = s_slot[q];
-        uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
-        const SceneGPU& S = scenes[views[view].scene];
-        float4 o = s_o[q], d = s_d[q], c = s_c[q];
-        HitRec h;
-        bool occluded = trace_ray<W, true, COUNT>(S, ld3(o), ld3(d), o.w, d.w, __float_as_uint(c.w), h, stk, &tc);
-        if (!occluded) {
-            float4 l = L[slot];
-            L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+    Tracer<LY, true, COUNT> tr;
+    uint32_t q = 0;
+    bool active = false;
+    for (;;) {
+        const uint32_t nq = qc.next(!active);
+        if (nq < n) {
+            const uint32_t slot = s_slot[nq];
+            const uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
+            const float4 o = s_o[nq], d = s_d[nq];
+            const uint32_t rflags = __float_as_uint(s_c[nq].w);
+            tr.begin(scenes[views[view].scene], ld3(o), ld3(d), o.w, d.w, rflags, &tc);
+            q = nq;
+            active = true;
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active && tr.template step<MODE>(stk)) {
+            if (!tr.h.found) {
+                // one shadow ray per path slot per bounce: no other lane adds to L[slot]
+                const uint32_t slot = s_slot[q];
+                const float4 c = s_c[q], l = L[slot];
+                L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+            }
+            active = false;
         }
     }
     if (COUNT) flush_counts(tc, tcounts + 3);
 }
 
-__global__ void k_reset2(uint32_t* a, uint32_t* b) {
+__global__ void k_reset4(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* d) {
     if (threadIdx.x == 0) {
         *a = 0;
         *b = 0;
+        *c = 0;
+        *d = 0;
     }
 }
 
@@ -713,7 +775,9 @@ struct ProfScope {
     }
 };
 
-template <int W, int BS, bool COUNT>
+// counters: [0], [1] extension queue lengths (ping-pong), [2] shadow queue
+// length, [4] extension queue head, [5] shadow queue head.
+template <int W, int BS, int MODE, bool COUNT>
 static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, const ViewGPU* d_views, const ViewGPU* h_views,
                              uint32_t nv, uint32_t total, uint32_t max_bounces, Wavefront& wf, bool mega) {
     hipStream_t st = cfg.stream;
@@ -721,34 +785,36 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     unsigned long long* tc = pf ? pf->d_counts : nullptr;
     const uint32_t grid_cap = (uint32_t)cfg.num_cu * (uint32_t)cfg.tune.blocks_per_cu;
     const uint32_t grid = std::max(1u, std::min(grid_cap, cdiv(total, BS)));
+    const uint32_t refill = cfg.tune.refill_lanes;
+    uint32_t* ctr = wf.counters;
     if (mega) {
         ProfScope ps(pf, PK_MEGAKERNEL, st);
         k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats, wf.spill);
     } else {
         {
             ProfScope ps(pf, PK_GENERATE, st);
-            k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng,
-                                                wf.counters);
+            k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, ctr);
         }
         for (uint32_t d = 0; d < max_bounces; d++) {
             const int a = d & 1, b = a ^ 1;
             {
                 ProfScope ps(pf, PK_EXTEND, st);
-                k_extend<W, BS, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
-                                                            wf.counters + a, d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
-                                                            wf.hit_tri, wf.stats, tc, wf.spill);
+                k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
+                                                                  ctr + a, ctr + 4, refill,
+                                                                  d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
+                                                                  wf.hit_tri, wf.stats, tc, wf.spill);
             }
-            k_reset2<<<1, 64, 0, st>>>(wf.counters + b, wf.counters + 2);
+            k_reset4<<<1, 64, 0, st>>>(ctr + b, ctr + 2, ctr + 4, ctr + 5);
             {
                 ProfScope ps(pf, PK_SHADE, st);
-                k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], wf.counters + a,
+                k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], ctr + a,
                                                  wf.hit, wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b],
-                                                 wf.counters + b, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, wf.counters + 2);
+                                                 ctr + b, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, ctr + 2);
             }
             {
                 ProfScope ps(pf, PK_SHADOW, st);
-                k_shadow<W, BS, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
-                                                            wf.counters + 2, wf.L, wf.stats, tc, wf.spill);
+                k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+                                                                  ctr + 2, ctr + 5, refill, wf.L, wf.stats, tc, wf.spill);
             }
         }
     }
@@ -767,23 +833,30 @@ hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const Sc
                         bool mega) {
     const int L = h_scenes[0].bvh_width;  // node layout id
     const int bs = cfg.tune.extend_block;
+    const bool ifif = cfg.tune.traversal == kIfIf;
+#define HH_FRAME(LL, BB, MM, CC) return frame_impl<LL, BB, MM, CC>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
     if (cfg.prof && cfg.prof->count && cfg.prof->d_counts) {
-        if (L == kLayoutB8) return frame_impl<kLayoutB8, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
-        if (L == kLayoutQ4) return frame_impl<kLayoutQ4, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
-        return frame_impl<kLayoutB4, 256, true>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega);
+        if (L == kLayoutB8) HH_FRAME(kLayoutB8, 256, kWhileWhile, true);
+        if (L == kLayoutQ4) HH_FRAME(kLayoutQ4, 256, kWhileWhile, true);
+        HH_FRAME(kLayoutB4, 256, kWhileWhile, true);
     }
-#define HH_DISPATCH(LL, BB) return frame_impl<LL, BB, false>(cfg, d_scenes, d_views, h_views, nv, total, max_bounces, wf, mega)
-#define HH_BLOCKS(LL)                      \
-    do {                                   \
-        if (bs == 64) HH_DISPATCH(LL, 64);   \
-        if (bs == 128) HH_DISPATCH(LL, 128); \
-        HH_DISPATCH(LL, 256);                \
+#define HH_MODES(LL, BB)                               \
+    do {                                               \
+        if (ifif) HH_FRAME(LL, BB, kIfIf, false);      \
+        HH_FRAME(LL, BB, kWhileWhile, false);          \
+    } while (0)
+#define HH_BLOCKS(LL)                       \
+    do {                                    \
+        if (bs == 64) HH_MODES(LL, 64);     \
+        if (bs == 128) HH_MODES(LL, 128);   \
+        HH_MODES(LL, 256);                  \
     } while (0)
     if (L == kLayoutB8) HH_BLOCKS(kLayoutB8);
     if (L == kLayoutQ4) HH_BLOCKS(kLayoutQ4);
     HH_BLOCKS(kLayoutB4);
 #undef HH_BLOCKS
-#undef HH_DISPATCH
+#undef HH_MODES
+#undef HH_FRAME
 }
 
 hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,

@@ -77,6 +77,8 @@ struct Tuning {
     int leaf_size = 4;
     uint32_t frames_in_flight = 4;
     bool bvh_compressed = false;
+    uint32_t refill_lanes = 32;  // idle lanes that make a traversal wave take new rays
+    int traversal = 0;           // 0 while-while, 1 if-if step shape
     int layout() const { return bvh_compressed && bvh_width == 4 ? kLayoutQ4 : bvh_width; }
 };
 

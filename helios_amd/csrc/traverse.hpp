@@ -192,7 +192,7 @@ __device__ inline float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k))
 // Tests the children of node `cur`: tn[k] = entry distance (+inf = miss or
 // absent child), ref[k] = child reference.
 template <int L>
-__device__ inline void node_test(const void* nodes, uint32_t cur, f3 o, f3 inv, float tmin, float tmax, float* tn,
+__device__ inline void node_test(const HH_GLOBAL void* nodes, uint32_t cur, f3 o, f3 inv, float tmin, float tmax, float* tn,
                                  uint32_t* ref) {
     if constexpr (L == kLayoutQ4) {
         const HH_GLOBAL uint4* p = (const HH_GLOBAL uint4*)nodes + 4 * (size_t)cur;
@@ -232,109 +232,198 @@ __device__ inline bool key_less(uint32_t i1, uint32_t g1, uint32_t p1, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// Traversal. ANY = stop at the first accepted hit (visibility rays).
-// rflags: HH_TRACE_OPAQUE (bit 0) disables the any-hit alpha test.
-// `stk` says where this lane's stack lives.
+// Traversal as a per-lane state machine. ANY = stop at the first accepted hit
+// (visibility rays). rflags: HH_TRACE_OPAQUE (bit 0) disables the any-hit
+// alpha test. `step` advances one lane; the persistent queue kernels interleave
+// steps with refills of finished lanes, so a wave never idles on the longest
+// ray of a batch. The result (closest (t, instance, geometry, primitive)) does
+// not depend on the visiting order, so every schedule is bit-identical.
 // ---------------------------------------------------------------------------
-template <int L, bool ANY, bool COUNT = false>
-__device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, float tmax, uint32_t rflags, HitRec& h,
-                                 const LaneStack& stk, TravCount* cnt = nullptr) {
-    constexpr int W = layout_width(L);
-    const HH_GLOBAL float4* tris = (const HH_GLOBAL float4*)S.tris;
-    HH_LDS uint32_t* const lds = stk.lds;
-    const uint32_t stride = stk.lds_stride;
-    f3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
-    f3 o = wo, inv = winv;
-    WT wt = wt_pre(wd);
-    int inst = -1;
-    uint32_t sp = 0;
-    h.found = false;
-    h.t = tmax;
-    h.u = h.v = 0.0f;
-    h.inst = 0xFFFFFFFFu;
-    h.tri = 0xFFFFFFFFu;
-    uint32_t hgeom = 0xFFFFFFFFu, hprim = 0xFFFFFFFFu;
-    const bool alpha_on = (rflags & 1u) == 0u;
+constexpr uint32_t kDone = 0xFFFFFFFEu;     // never a valid child reference
+constexpr uint32_t kMaxTrips = 1u << 22;    // hard bound on node + leaf visits per ray
+
+// Step shapes: if-if visits one node or leaf per step; while-while visits
+// inner nodes until the lane reaches a leaf (or finishes), then one leaf.
+enum TravMode { kWhileWhile = 0, kIfIf = 1 };
+
+template <int L, bool ANY, bool COUNT>
+struct Tracer {
+    static constexpr int W = layout_width(L);
+    const SceneGPU* S;
+    // scene arrays read on every visit, held in registers: reading them
+    // through S would put a dependent load in front of every node fetch
+    const HH_GLOBAL void* nodes;
+    const HH_GLOBAL float4* tris;
+    const HH_GLOBAL InstTrav* insts;
+    f3 wo, wd, winv;   // world-space ray
+    f3 o, inv;         // ray in the current (world or instance) space
+    WT wt;
+    float tmin;
+    int inst;          // instance being traversed, -1 at TLAS level
+    uint32_t sp, cur, trips;
+    bool alpha_on;
+    HitRec h;
+    uint32_t hgeom, hprim;
+    TravCount* cnt;
+
+    __device__ inline void begin(const SceneGPU& scene, f3 ro, f3 rd, float t0, float t1, uint32_t rflags,
+                                 TravCount* c = nullptr) {
+        S = &scene;
+        nodes = (const HH_GLOBAL void*)scene.nodes;
+        tris = (const HH_GLOBAL float4*)scene.tris;
+        insts = (const HH_GLOBAL InstTrav*)scene.inst_trav;
+        wo = ro;
+        wd = rd;
+        winv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        o = wo;
+        inv = winv;
+        wt = wt_pre(rd);
+        tmin = t0;
+        inst = -1;
+        sp = 0;
+        trips = 0;
+        alpha_on = (rflags & 1u) == 0u;
+        h.found = false;
+        h.t = t1;
+        h.u = h.v = 0.0f;
+        h.inst = 0xFFFFFFFFu;
+        h.tri = 0xFFFFFFFFu;
+        hgeom = hprim = 0xFFFFFFFFu;
+        cnt = c;
+        cur = scene.tlas_root;
+    }
+
     // Entries beyond the stack capacity are dropped (never written out of
     // bounds); the capacity is far above the depth of the built trees.
-    auto push = [&](uint32_t r) {
-        if (sp < (uint32_t)kLdsStack) lds[sp * stride] = r;
+    __device__ inline void push(const LaneStack& stk, uint32_t r) {
+        if (sp < (uint32_t)kLdsStack) stk.lds[sp * stk.lds_stride] = r;
         else if (sp < (uint32_t)(kLdsStack + kSpillStack)) stk.spill[(sp - kLdsStack) * stk.spill_stride] = r;
         else return;
         sp++;
-    };
-    uint32_t cur = S.tlas_root;
-    // Hard bound on loop trips: every wave leaves the loop even if a
-    // malformed tree contained a cycle.
-    for (uint32_t trips = 0; trips < (1u << 22); trips++) {
-        if ((cur & kLeafBit) == 0u) {
-            if (COUNT) cnt->nodes++;
-            float tn[W];
-            uint32_t ref[W];
-            node_test<L>(S.nodes, cur, o, inv, tmin, h.t, tn, ref);
-            // sort ascending by entry distance (small network, registers only)
+    }
+    __device__ inline void pop(const LaneStack& stk) {
+        if (sp == 0) {
+            cur = kDone;
+            return;
+        }
+        sp--;
+        cur = sp < (uint32_t)kLdsStack ? stk.lds[sp * stk.lds_stride] : stk.spill[(sp - kLdsStack) * stk.spill_stride];
+    }
+
+    // Children of inner node `cur` in front-to-back order: descend into the
+    // nearest, push the others, or pop when none is hit.
+    __device__ inline void visit_inner(const LaneStack& stk) {
+        if (COUNT) cnt->nodes++;
+        float tn[W];
+        uint32_t ref[W];
+        node_test<L>(nodes, cur, o, inv, tmin, h.t, tn, ref);
+        if (ANY) {
+            // any accepted hit ends the ray, so the order is free: children
+            // in node order (measured to find occluders in fewer visits
+            // than nearest-first on the sun and point-light rays)
 #pragma unroll
-            for (int i = 1; i < W; i++)
+            for (int k = W - 1; k >= 0; k--)
+                if (tn[k] != INFINITY) push(stk, ref[k]);
+            pop(stk);
+            return;
+        }
 #pragma unroll
-                for (int j = W - 1; j >= i; j--)
-                    if (tn[j] < tn[j - 1]) {
-                        float tt = tn[j]; tn[j] = tn[j - 1]; tn[j - 1] = tt;
-                        uint32_t rr = ref[j]; ref[j] = ref[j - 1]; ref[j - 1] = rr;
-                    }
-            if (tn[0] == INFINITY) goto pop;
+        for (int i = 1; i < W; i++)
 #pragma unroll
-            for (int k = W - 1; k >= 1; k--)
-                if (tn[k] != INFINITY) push(ref[k]);
-            cur = ref[0];
-            continue;
-        } else if (cur == kMarker) {
+            for (int j = W - 1; j >= i; j--)
+                if (tn[j] < tn[j - 1]) {
+                    float tt = tn[j]; tn[j] = tn[j - 1]; tn[j - 1] = tt;
+                    uint32_t rr = ref[j]; ref[j] = ref[j - 1]; ref[j - 1] = rr;
+                }
+        if (tn[0] == INFINITY) {
+            pop(stk);
+            return;
+        }
+#pragma unroll
+        for (int k = W - 1; k >= 1; k--)
+            if (tn[k] != INFINITY) push(stk, ref[k]);
+        cur = ref[0];
+    }
+
+    // Triangles of leaf `cur` (BLAS level); returns true when an any-hit ray
+    // found its occluder.
+    __device__ inline bool visit_leaf() {
+        const uint32_t first = leaf_start(cur), ntri = leaf_count(cur);
+        for (uint32_t k = 0; k < ntri; k++) {
+            const uint32_t ti = first + k;
+            const float4 a = tris[3 * ti], b = tris[3 * ti + 1], c = tris[3 * ti + 2];
+            if (COUNT) cnt->tris++;
+            float t, u, v;
+            if (!tri_hit(wt, o, ld3(a), ld3(b), ld3(c), tmin, h.t, t, u, v)) continue;
+            const uint32_t prim = __float_as_uint(a.w);
+            const uint32_t gw = __float_as_uint(b.w);
+            const uint32_t geom = gw & 0x7FFFFFFFu;
+            if (h.found && t == h.t && !key_less((uint32_t)inst, geom, prim, h.inst, hgeom, hprim)) continue;
+            if (alpha_on && (gw & 0x80000000u) == 0u) {
+                if (hit_alpha(*S, (uint32_t)inst, geom, prim, u, v) < 0.1f) continue;
+            }
+            h.found = true;
+            h.t = t;
+            h.u = u;
+            h.v = v;
+            h.inst = (uint32_t)inst;
+            h.tri = ti;
+            hgeom = geom;
+            hprim = prim;
+            if (ANY) return true;
+        }
+        return false;
+    }
+
+    // Advances the traversal; returns true once the ray is finished (h holds
+    // the result).
+    template <int MODE>
+    __device__ inline bool step(const LaneStack& stk) {
+        if (MODE == kWhileWhile) {
+            while ((cur & kLeafBit) == 0u && trips < kMaxTrips) {
+                trips++;
+                visit_inner(stk);
+            }
+        } else if ((cur & kLeafBit) == 0u && trips < kMaxTrips) {
+            trips++;
+            visit_inner(stk);
+            return false;
+        }
+        if (cur == kDone || trips >= kMaxTrips) return true;
+        trips++;
+        if (cur == kMarker) {
             o = wo;
             inv = winv;
             inst = -1;
-            goto pop;
+            pop(stk);
         } else if (inst < 0) {
             inst = (int)leaf_start(cur);
-            const HH_GLOBAL InstTrav& it = ((const HH_GLOBAL InstTrav*)S.inst_trav)[inst];
+            const HH_GLOBAL InstTrav& it = insts[inst];
             if (COUNT) cnt->insts++;
             f3 od;
             xform_ray(it.w2o, wo, wd, o, od);
             inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
             wt = wt_pre(od);
-            push(kMarker);
+            push(stk, kMarker);
             cur = it.blas_root;
-            continue;
+            return false;
         } else {
-            const uint32_t first = leaf_start(cur), ntri = leaf_count(cur);
-            for (uint32_t k = 0; k < ntri; k++) {
-                const uint32_t ti = first + k;
-                float4 a = tris[3 * ti], b = tris[3 * ti + 1], c = tris[3 * ti + 2];
-                if (COUNT) cnt->tris++;
-                float t, u, v;
-                if (!tri_hit(wt, o, ld3(a), ld3(b), ld3(c), tmin, h.t, t, u, v)) continue;
-                const uint32_t prim = __float_as_uint(a.w);
-                const uint32_t gw = __float_as_uint(b.w);
-                const uint32_t geom = gw & 0x7FFFFFFFu;
-                if (h.found && t == h.t && !key_less((uint32_t)inst, geom, prim, h.inst, hgeom, hprim)) continue;
-                if (alpha_on && (gw & 0x80000000u) == 0u) {
-                    if (hit_alpha(S, (uint32_t)inst, geom, prim, u, v) < 0.1f) continue;
-                }
-                h.found = true;
-                h.t = t;
-                h.u = u;
-                h.v = v;
-                h.inst = (uint32_t)inst;
-                h.tri = ti;
-                hgeom = geom;
-                hprim = prim;
-                if (ANY) return true;
-            }
-            goto pop;
+            if (visit_leaf()) return true;
+            pop(stk);
         }
-    pop:
-        if (sp == 0) break;
-        sp--;
-        cur = sp < (uint32_t)kLdsStack ? lds[sp * stride] : stk.spill[(sp - kLdsStack) * stk.spill_stride];
+        return cur == kDone;
     }
+};
+
+// Whole traversal of one ray in one call (debug entry points, megakernel).
+template <int L, bool ANY, bool COUNT = false>
+__device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, float tmax, uint32_t rflags, HitRec& h,
+                                 const LaneStack& stk, TravCount* cnt = nullptr) {
+    Tracer<L, ANY, COUNT> tr;
+    tr.begin(S, wo, wd, tmin, tmax, rflags, cnt);
+    while (!tr.template step<kWhileWhile>(stk)) {
+    }
+    h = tr.h;
     return h.found;
 }
 

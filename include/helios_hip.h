@@ -341,6 +341,14 @@ typedef struct hh_tuning {
     /* 1 = quantized 64-B BVH4 nodes (8-bit child boxes, outward rounded);
      * applies to bvh_width 4 and takes effect at the next scene commit. */
     uint32_t bvh_compressed;
+    /* Persistent traversal kernels keep one ray per lane in flight and take
+     * new rays from the queue for a wave once this many of its 64 lanes have
+     * finished (1..64; 64 = only when the whole wave is idle). */
+    uint32_t refill_lanes;
+    /* Step shape of the traversal loop: 0 = while-while (inner nodes until
+     * the lane holds a leaf, then the leaf), 1 = if-if (one node or leaf per
+     * step). Results are identical; only the schedule differs. */
+    uint32_t traversal;
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);
