@@ -133,6 +133,7 @@ struct hh_surface_t {
     hh_context ctx = nullptr;
     uint32_t w = 0, h = 0;
     float4* d = nullptr;
+    float4* aux = nullptr;  // two feature planes (hh_surface_enable_features)
 };
 
 struct hh_integrator_t {
@@ -1037,6 +1038,7 @@ hh_status hh_surface_destroy(hh_surface s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     s->ctx->drop_graphs();
     (void)hipFree(s->d);
+    if (s->aux) (void)hipFree(s->aux);
     delete s;
     return HH_OK;
 }
@@ -1044,6 +1046,26 @@ hh_status hh_surface_destroy(hh_surface s) {
 hh_status hh_surface_clear(hh_surface s) {  // vkCmdClearColorImage (renderer.cpp:212-223)
     HH_ARG(s, "null surface");
     HH_HIP(launch_fill_f4(s->ctx->stream, s->d, make_float4(0.f, 0.f, 0.f, 1.f), s->w * s->h));
+    if (s->aux) HH_HIP(launch_fill_f4(s->ctx->stream, s->aux, make_float4(0.f, 0.f, 0.f, 0.f), 2 * s->w * s->h));
+    return HH_OK;
+}
+
+hh_status hh_surface_enable_features(hh_surface s) {
+    HH_ARG(s, "null surface");
+    if (s->aux) return HH_OK;
+    HH_HIP(hipSetDevice(s->ctx->device));
+    HH_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->ctx->drop_graphs();
+    HH_HIP(hipMalloc(&s->aux, 2 * sizeof(float4) * s->w * s->h));
+    HH_HIP(launch_fill_f4(s->ctx->stream, s->aux, make_float4(0.f, 0.f, 0.f, 0.f), 2 * s->w * s->h));
+    return HH_OK;
+}
+
+hh_status hh_surface_read_features(hh_surface s, float* out) {
+    HH_ARG(s && out, "null argument");
+    HH_ARG(s->aux, "the surface has no feature planes (hh_surface_enable_features)");
+    HH_HIP(hipMemcpyAsync(out, s->aux, 2 * sizeof(float4) * s->w * s->h, hipMemcpyDeviceToHost, s->ctx->stream));
+    HH_HIP(hipStreamSynchronize(s->ctx->stream));
     return HH_OK;
 }
 
@@ -1241,7 +1263,10 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
     // samples per wavefront pass (never crossing a tile's max_samples).
     uint32_t f = 0;
     while (f < frames) {
-        if (it->num_acc == 0 && it->tile_idx == 0) HH_HIP(launch_fill_f4(ctx->stream, out->d, make_float4(0.f, 0.f, 0.f, 1.f), out->w * out->h));
+        if (it->num_acc == 0 && it->tile_idx == 0) {
+            HH_HIP(launch_fill_f4(ctx->stream, out->d, make_float4(0.f, 0.f, 0.f, 1.f), out->w * out->h));
+            if (out->aux) HH_HIP(launch_fill_f4(ctx->stream, out->aux, make_float4(0.f, 0.f, 0.f, 0.f), 2 * out->w * out->h));
+        }
         if (it->tile_idx >= it->tiles.size()) {
             f++;
             continue;
@@ -1262,6 +1287,7 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
         v.scene = 0;
         v.slot_base = 0;
         v.accum = out->d;
+        v.aux = out->aux;
         std::vector<ViewGPU> views{v};
         st = run_frame(ctx, scenes, views, v.tile_w * v.tile_h * chunk, it->s.max_ray_bounces, mega,
                        std::to_string(scene->generation) + "|" + std::to_string((uintptr_t)out->d));
@@ -1358,6 +1384,7 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
             v.scene = scene_index[views[i].scene];
             v.slot_base = total;
             v.accum = sf->d;
+            v.aux = sf->aux;
             total += sf->w * sf->h;
             maxb = std::max(maxb, v.max_bounces);
             vg.push_back(v);
@@ -1475,6 +1502,7 @@ hipError_t hh::Wavefront::ensure(uint32_t n) {
     HH_TRY(hipMalloc(&s_d, N * 16));
     HH_TRY(hipMalloc(&s_c, N * 16));
     HH_TRY(hipMalloc(&s_slot, N * 4));
+    HH_TRY(hipMalloc(&feat, N * 32));
     HH_TRY(hipMalloc(&counters, 64));
     HH_TRY(hipMalloc(&stats, 64));
     capacity = (uint32_t)N;
@@ -1495,7 +1523,7 @@ void hh::Wavefront::release() {
     if (spill) (void)hipFree(spill);
     spill = nullptr;
     spill_threads = 0;
-    void* ps[] = {T, L, rng, q_o[0], q_o[1], q_d[0], q_d[1], q_slot[0], q_slot[1], hit, hit_tri, s_o, s_d, s_c, s_slot, counters, stats};
+    void* ps[] = {T, L, rng, q_o[0], q_o[1], q_d[0], q_d[1], q_slot[0], q_slot[1], hit, hit_tri, s_o, s_d, s_c, s_slot, feat, counters, stats};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     T = L = nullptr;
@@ -1506,6 +1534,7 @@ void hh::Wavefront::release() {
     hit_tri = nullptr;
     s_o = s_d = s_c = nullptr;
     s_slot = nullptr;
+    feat = nullptr;
     counters = nullptr;
     stats = nullptr;
     capacity = 0;

@@ -146,6 +146,10 @@ struct ViewGPU {
     uint32_t scene;      // index into the scene table
     uint32_t slot_base;  // first path slot of this view
     float4* accum;       // RGBA32F surface
+    // Optional feature planes of the surface (nullptr = none): aux[p] =
+    // (albedo rgb, 1/(1+t)) and aux[width*height + p] = (shading normal xyz,
+    // M2 of the sample luminance), accumulated like the radiance.
+    float4* aux;
 };
 
 }  // namespace hh

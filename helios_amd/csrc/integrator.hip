@@ -293,6 +293,17 @@ __device__ inline ShadowRay direct_lighting(const SceneGPU& S, const Surface& p,
     return sr;
 }
 
+// First-hit features of one sample (denoiser inputs): albedo (the clamped
+// environment radiance on a miss), depth as 1/(1+t) (0 on a miss), shading
+// normal (0 on a miss).
+__device__ inline void write_features(float4* feat, uint32_t stride, uint32_t slot, f3 albedo, float t, f3 n, bool miss) {
+    if (miss) albedo = mk(fminf(albedo.x, 1.0f), fminf(albedo.y, 1.0f), fminf(albedo.z, 1.0f));
+    feat[slot] = make_float4(albedo.x, albedo.y, albedo.z, miss ? 0.0f : 1.0f / (1.0f + t));
+    feat[stride + slot] = make_float4(n.x, n.y, n.z, 0.0f);
+}
+
+__device__ inline float luminance(float3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
@@ -418,7 +429,7 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
                                               float4* __restrict__ n_o, float4* __restrict__ n_d, uint32_t* __restrict__ n_slot,
                                               uint32_t* __restrict__ n_count, float4* __restrict__ s_o, float4* __restrict__ s_d,
                                               float4* __restrict__ s_c, uint32_t* __restrict__ s_slot,
-                                              uint32_t* __restrict__ s_count) {
+                                              uint32_t* __restrict__ s_count, float4* __restrict__ feat, uint32_t feat_stride) {
     const uint32_t n = *count_p;
     const uint32_t stride = gridDim.x * BS;
     // Keep whole waves in the loop so the ballot-based compaction sees every
@@ -444,13 +455,16 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
             f3 Lp = ld3(L4);
             uint2 rs = rngs[slot];
             RNG rng{rs.x, rs.y};
+            const bool features = depth == 0 && V.aux != nullptr;
             if (hh.x < 0.0f) {
                 // miss (path_trace_rmiss.glsl:60-65)
                 f3 e = sample_env(S, dir);
                 Lp = Lp + (depth == 0 ? e : Tp * e);
+                if (features) write_features(feat, feat_stride, slot, e, 0.0f, mk(0.f, 0.f, 0.f), true);
             } else {
                 Surface p;
                 populate(S, __float_as_uint(hh.w), hit_tri[q], hh.y, hh.z, p);
+                if (features) write_features(feat, feat_stride, slot, p.albedo, hh.x, p.normal, false);
                 if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
                 sr = direct_lighting(S, p, dir, Tp, depth, V.shadow_bias, rng);
                 if ((V.flags & HH_FLAG_DIRECT_ONLY_BIT) == 0u && depth + 1 < V.max_bounces) {
@@ -542,26 +556,57 @@ __global__ void k_reset4(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* d) {
 }
 
 // path_trace_rgen.glsl:219-247: clamp and progressive mean, samples in order.
-__global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, const float4* __restrict__ L) {
+// The feature planes follow the same recurrence, so they average the same
+// samples as the radiance (samples 1..f after frame f); M2 is Welford's sum
+// of squared luminance deviations about that running mean.
+__global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, const float4* __restrict__ L,
+                             const float4* __restrict__ feat, uint32_t feat_stride) {
     const ViewGPU& v = views[blockIdx.y];
     const uint32_t npix = v.tile_w * v.tile_h;
+    float4* const aux = v.aux;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
         uint32_t y = v.tile_y + p / v.tile_w, x = v.tile_x + p % v.tile_w;
-        float4* px = v.accum + (size_t)y * v.width + x;
+        const size_t pi = (size_t)y * v.width + x;
+        float4* px = v.accum + pi;
         float4 prev = *px;
+        float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
+        if (aux) {
+            a0 = aux[pi];
+            a1 = aux[(size_t)v.width * v.height + pi];
+        }
         for (uint32_t s = 0; s < v.spp; s++) {
-            float4 l = L[v.slot_base + s * npix + p];
+            const uint32_t slot = v.slot_base + s * npix + p;
+            float4 l = L[slot];
             float3 c;
             if (v.flags & HH_FLAG_NO_CLAMP_BIT) c = make_float3(l.x, l.y, l.z);
             else c = make_float3(fminf(l.x, 1.0f), fminf(l.y, 1.0f), fminf(l.z, 1.0f));
             uint32_t f = v.first_frame + s;
+            const float lum_prev = luminance(make_float3(prev.x, prev.y, prev.z));
             if (f == 0) prev = make_float4(c.x, c.y, c.z, 1.0f);
             else {
                 float ff = (float)f;
                 prev = make_float4(prev.x + (c.x - prev.x) / ff, prev.y + (c.y - prev.y) / ff, prev.z + (c.z - prev.z) / ff, 1.0f);
             }
+            if (aux) {
+                const float4 g0 = feat[slot], g1 = feat[feat_stride + slot];
+                const float xl = luminance(c);
+                if (f == 0) {
+                    a0 = g0;
+                    a1 = make_float4(g1.x, g1.y, g1.z, 0.0f);
+                } else {
+                    const float ff = (float)f;
+                    a0 = make_float4(a0.x + (g0.x - a0.x) / ff, a0.y + (g0.y - a0.y) / ff, a0.z + (g0.z - a0.z) / ff,
+                                     a0.w + (g0.w - a0.w) / ff);
+                    const float m2 = f == 1 ? 0.0f : a1.w + (xl - lum_prev) * (xl - luminance(make_float3(prev.x, prev.y, prev.z)));
+                    a1 = make_float4(a1.x + (g1.x - a1.x) / ff, a1.y + (g1.y - a1.y) / ff, a1.z + (g1.z - a1.z) / ff, m2);
+                }
+            }
         }
         *px = prev;
+        if (aux) {
+            aux[pi] = a0;
+            aux[(size_t)v.width * v.height + pi] = a1;
+        }
     }
 }
 
@@ -569,7 +614,8 @@ __global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, con
 template <int W, int BS>
 __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
                                                    uint32_t nv, uint32_t total, float4* __restrict__ Lout,
-                                                   unsigned long long* __restrict__ stats, uint32_t* __restrict__ spill) {
+                                                   unsigned long long* __restrict__ stats, uint32_t* __restrict__ spill,
+                                                   float4* __restrict__ feat, uint32_t feat_stride) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     const uint32_t stride = gridDim.x * BS;
@@ -588,13 +634,16 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
             HitRec h;
             n_ext++;
             trace_ray<W, false>(S, o, d, tmin, 10000.0f, rflags, h, stk);
+            const bool features = depth == 0 && V.aux != nullptr;
             if (!h.found) {
                 f3 e = sample_env(S, d);
                 Lp = Lp + (depth == 0 ? e : T * e);
+                if (features) write_features(feat, feat_stride, slot, e, 0.0f, mk(0.f, 0.f, 0.f), true);
                 break;
             }
             Surface p;
             populate(S, h.inst, h.tri, h.u, h.v, p);
+            if (features) write_features(feat, feat_stride, slot, p.albedo, h.t, p.normal, false);
             if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
             ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, rng);
             if (sr.want) {
@@ -789,7 +838,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     uint32_t* ctr = wf.counters;
     if (mega) {
         ProfScope ps(pf, PK_MEGAKERNEL, st);
-        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats, wf.spill);
+        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats, wf.spill, wf.feat, wf.capacity);
     } else {
         {
             ProfScope ps(pf, PK_GENERATE, st);
@@ -809,7 +858,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
                 ProfScope ps(pf, PK_SHADE, st);
                 k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], ctr + a,
                                                  wf.hit, wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b],
-                                                 ctr + b, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, ctr + 2);
+                                                 ctr + b, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, ctr + 2, wf.feat, wf.capacity);
             }
             {
                 ProfScope ps(pf, PK_SHADOW, st);
@@ -823,7 +872,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     dim3 ag(std::max(1u, std::min(cdiv(maxpix, 256), 1024u)), nv);
     {
         ProfScope ps(pf, PK_ACCUMULATE, st);
-        k_accumulate<<<ag, 256, 0, st>>>(d_views, nv, wf.L);
+        k_accumulate<<<ag, 256, 0, st>>>(d_views, nv, wf.L, wf.feat, wf.capacity);
     }
     return hipGetLastError();
 }

@@ -56,6 +56,9 @@ struct Wavefront {
     float4* s_d = nullptr;
     float4* s_c = nullptr;       // contribution rgb, bits(ray flags)
     uint32_t* s_slot = nullptr;
+    // first-hit features by slot, written at depth 0 when a view has feature
+    // planes: feat[slot] = (albedo, 1/(1+t)), feat[capacity + slot] = (normal, 0)
+    float4* feat = nullptr;
     // counters: [0,1] extension queue sizes, [2] shadow size, [3..] stats
     uint32_t* counters = nullptr;
     unsigned long long* stats = nullptr;  // [0] extension rays, [1] shadow rays

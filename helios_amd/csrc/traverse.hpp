@@ -339,6 +339,9 @@ struct Tracer {
             pop(stk);
             return;
         }
+        // (a branch-free variant that stores every child, sending unused
+        // ones to a discard slot, measured slower: it raised register
+        // pressure past the six-wave budget)
 #pragma unroll
         for (int k = W - 1; k >= 1; k--)
             if (tn[k] != INFINITY) push(stk, ref[k]);

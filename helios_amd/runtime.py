@@ -62,6 +62,8 @@ _SIGS = {
     "hh_surface_read": [C.c_void_p, C.c_void_p],
     "hh_surface_write": [C.c_void_p, C.c_void_p],
     "hh_surface_device_ptr": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "hh_surface_enable_features": [C.c_void_p],
+    "hh_surface_read_features": [C.c_void_p, C.c_void_p],
     "hh_tonemap": [C.c_void_p, C.c_float, C.c_uint32, C.c_void_p],
     "hh_integrator_create": [C.c_void_p, C.POINTER(C.c_void_p)],
     "hh_integrator_destroy": [C.c_void_p],
@@ -288,6 +290,15 @@ class Surface:
     def write(self, img):
         img = np.ascontiguousarray(img, np.float32)
         check(lib().hh_surface_write(self.h, ptr(img)))
+
+    def enable_features(self):
+        check(lib().hh_surface_enable_features(self.h))
+
+    def read_features(self):
+        """(albedo rgb + 1/(1+t), normal xyz + M2) planes, each (H, W, 4)."""
+        out = np.zeros((2, self.height, self.width, 4), np.float32)
+        check(lib().hh_surface_read_features(self.h, ptr(out)))
+        return out[0], out[1]
 
     def device_ptr(self):
         p = C.c_void_p()

@@ -271,6 +271,15 @@ hh_status hh_surface_clear(hh_surface s);  /* vkCmdClearColorImage to (0,0,0,1) 
 hh_status hh_surface_read(hh_surface s, float* out_rgba);
 hh_status hh_surface_write(hh_surface s, const float* rgba);
 hh_status hh_surface_device_ptr(hh_surface s, void** out);
+/* First-hit feature planes for the denoiser (no counterpart in the
+ * reference). Once enabled, every progressive frame also accumulates, with
+ * the radiance's own recurrence, plane 0 = (albedo rgb, 1/(1+t)) and plane 1
+ * = (shading normal xyz, M2), M2 being Welford's sum of squared deviations
+ * of the clamped sample luminance (Rec. 709) about the running mean. On a
+ * miss: albedo = environment radiance clamped to 1, depth 0, normal 0.
+ * hh_surface_read_features writes plane 0 then plane 1 (2 x W*H x 4 floats). */
+hh_status hh_surface_enable_features(hh_surface s);
+hh_status hh_surface_read_features(hh_surface s, float* out);
 /* tone_map.frag: exposure -> ACES (op 0) / Reinhard (op 1) / none -> gamma
  * 1/2.2 -> RGBA8 (row 0 = bottom, as stored). */
 hh_status hh_tonemap(hh_surface s, float exposure, uint32_t op, uint8_t* out_rgba8);
