@@ -22,7 +22,7 @@ LIB = PKG / "libhelios_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "bvh_build.hip", "integrator.hip", "sky_host.hip"]
+SOURCES = ["api.hip", "bvh_build.hip", "integrator.hip", "sky_host.hip", "denoise.hip"]
 
 
 def flags():

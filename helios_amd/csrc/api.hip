@@ -90,6 +90,18 @@ struct hh_context_t {
     }
 };
 
+struct hh_denoiser_t {
+    hh_context ctx = nullptr;
+    dn::Net net;
+    float* d_feat = nullptr;     // staged raw records
+    float* d_target = nullptr;   // staged targets
+    size_t stage_px = 0;
+    ~hh_denoiser_t() {
+        if (d_feat) (void)hipFree(d_feat);
+        if (d_target) (void)hipFree(d_target);
+    }
+};
+
 struct TexHost {
     uint32_t w = 0, h = 0, fmt = 0;
     std::vector<float> texels;
@@ -1476,6 +1488,149 @@ hh_status hh_primary_hits(hh_scene s, const hh_camera* cam, uint32_t w, uint32_t
     HH_HIP(hipMemcpyAsync(ids.data(), d_ids, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     HH_HIP(hipStreamSynchronize(ctx->stream));
     to_hits(hit, ids, out, n);
+    return HH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Denoiser
+// ---------------------------------------------------------------------------
+hh_status hh_denoiser_num_params(uint64_t* out) {
+    HH_ARG(out, "null argument");
+    *out = dn::net_num_params();
+    return HH_OK;
+}
+
+hh_status hh_denoiser_create(hh_context ctx, uint64_t seed, hh_denoiser* out) {
+    HH_ARG(ctx && out, "null argument");
+    HH_HIP(hipSetDevice(ctx->device));
+    hh_denoiser d = new (std::nothrow) hh_denoiser_t();
+    if (!d) return fail(HH_ERR_OUT_OF_MEMORY, "denoiser");
+    d->ctx = ctx;
+    hipError_t e = d->net.init();
+    if (e != hipSuccess) {
+        delete d;
+        return hip_fail(e, "denoiser allocation");
+    }
+    std::vector<float> p(d->net.nparams, 0.0f);
+    dn::net_he_init(seed, p.data());
+    hh_status st = hh_denoiser_set_params(d, p.data(), p.size());
+    if (st != HH_OK) {
+        delete d;
+        return st;
+    }
+    *out = d;
+    return HH_OK;
+}
+
+hh_status hh_denoiser_destroy(hh_denoiser d) {
+    if (!d) return HH_OK;
+    (void)hipStreamSynchronize(d->ctx->stream);
+    delete d;
+    return HH_OK;
+}
+
+hh_status hh_denoiser_set_params(hh_denoiser d, const float* params, uint64_t n) {
+    HH_ARG(d && params, "null argument");
+    HH_ARG(n == d->net.nparams, "parameter count mismatch");
+    hipStream_t st = d->ctx->stream;
+    HH_HIP(hipMemcpyAsync(d->net.master, params, n * 4, hipMemcpyHostToDevice, st));
+    HH_HIP(d->net.zero_state(st));
+    HH_HIP(d->net.refresh_weights(st));
+    HH_HIP(hipStreamSynchronize(st));
+    return HH_OK;
+}
+
+static hh_status dn_read(hh_denoiser d, const float* src, float* out, uint64_t n) {
+    HH_ARG(d && out, "null argument");
+    HH_ARG(n == d->net.nparams, "parameter count mismatch");
+    HH_HIP(hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, d->ctx->stream));
+    HH_HIP(hipStreamSynchronize(d->ctx->stream));
+    return HH_OK;
+}
+hh_status hh_denoiser_get_params(hh_denoiser d, float* out, uint64_t n) { return dn_read(d, d ? d->net.master : nullptr, out, n); }
+hh_status hh_denoiser_get_grads(hh_denoiser d, float* out, uint64_t n) { return dn_read(d, d ? d->net.grad : nullptr, out, n); }
+
+static uint32_t pad8(uint32_t v) { return (v + 7u) & ~7u; }
+
+// Uploads raw records (and targets) and packs the network input.
+static hh_status dn_stage(hh_denoiser d, const float* feats, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                          bool training) {
+    HH_ARG(d && feats, "null argument");
+    HH_ARG(n >= 1 && h >= 1 && w >= 1 && (size_t)n * pad8(h) * pad8(w) <= (size_t(1) << 28), "bad batch size");
+    hipStream_t st = d->ctx->stream;
+    HH_HIP(hipSetDevice(d->ctx->device));
+    const size_t px = (size_t)n * h * w;
+    if (px > d->stage_px) {
+        if (d->d_feat) (void)hipFree(d->d_feat);
+        if (d->d_target) (void)hipFree(d->d_target);
+        d->d_feat = d->d_target = nullptr;
+        d->stage_px = 0;
+        HH_HIP(hipMalloc(&d->d_feat, px * 12 * 4));
+        HH_HIP(hipMalloc(&d->d_target, px * 3 * 4));
+        d->stage_px = px;
+    }
+    HH_HIP(d->net.ensure(n, pad8(h), pad8(w), training));
+    HH_HIP(hipMemcpyAsync(d->d_feat, feats, px * 12 * 4, hipMemcpyHostToDevice, st));
+    if (target) HH_HIP(hipMemcpyAsync(d->d_target, target, px * 3 * 4, hipMemcpyHostToDevice, st));
+    HH_HIP(d->net.pack(st, d->d_feat, nullptr, nullptr, 0, h, w));
+    return HH_OK;
+}
+
+hh_status hh_denoiser_forward(hh_denoiser d, const float* feats, uint32_t n, uint32_t h, uint32_t w, float* out_log) {
+    HH_ARG(out_log, "null argument");
+    hh_status s = dn_stage(d, feats, nullptr, n, h, w, false);
+    if (s != HH_OK) return s;
+    hipStream_t st = d->ctx->stream;
+    HH_HIP(d->net.forward(st, nullptr, h, w));
+    const uint32_t Hp = d->net.H, Wp = d->net.W;
+    std::vector<float> o((size_t)n * Hp * Wp * 4);
+    HH_HIP(hipMemcpyAsync(o.data(), d->net.out, o.size() * 4, hipMemcpyDeviceToHost, st));
+    HH_HIP(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t y = 0; y < h; y++)
+            for (uint32_t x = 0; x < w; x++)
+                for (int c = 0; c < 3; c++)
+                    out_log[(((size_t)i * h + y) * w + x) * 3 + c] = o[(((size_t)i * Hp + y) * Wp + x) * 4 + c];
+    return HH_OK;
+}
+
+hh_status hh_denoiser_loss_grad(hh_denoiser d, const float* feats, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                                float* loss) {
+    HH_ARG(target, "null target");
+    hh_status s = dn_stage(d, feats, target, n, h, w, true);
+    if (s != HH_OK) return s;
+    hipStream_t st = d->ctx->stream;
+    HH_HIP(d->net.forward(st, nullptr, h, w));
+    HH_HIP(d->net.loss_and_backward(st, d->d_target, h, w));
+    float l = 0.0f;
+    HH_HIP(hipMemcpyAsync(&l, d->net.loss, 4, hipMemcpyDeviceToHost, st));
+    HH_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = l / (3.0f * (float)n * (float)h * (float)w);
+    return HH_OK;
+}
+
+hh_status hh_denoiser_train_step(hh_denoiser d, const float* feats, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                                 float lr, float* loss) {
+    HH_ARG(std::isfinite(lr) && lr >= 0.0f, "learning rate must be finite and >= 0");
+    hh_status s = hh_denoiser_loss_grad(d, feats, target, n, h, w, loss);
+    if (s != HH_OK) return s;
+    HH_HIP(d->net.adam(d->ctx->stream, lr));
+    return HH_OK;
+}
+
+hh_status hh_denoise(hh_denoiser d, hh_surface in, uint32_t num_frames, hh_surface out) {
+    HH_ARG(d && in && out, "null argument");
+    HH_ARG(in->aux, "the input surface has no feature planes (hh_surface_enable_features)");
+    HH_ARG(in->w == out->w && in->h == out->h, "surface sizes differ");
+    HH_ARG(in != out, "denoise into a different surface");
+    HH_ARG(num_frames >= 1, "num_frames must be >= 1");
+    HH_ARG(in->ctx == d->ctx && out->ctx == d->ctx, "objects belong to different contexts");
+    hipStream_t st = d->ctx->stream;
+    HH_HIP(hipSetDevice(d->ctx->device));
+    HH_HIP(d->net.ensure(1, pad8(in->h), pad8(in->w), false));
+    const uint32_t held = num_frames >= 2 ? num_frames - 1 : 1;
+    HH_HIP(d->net.pack(st, nullptr, in->d, in->aux, held, in->h, in->w));
+    HH_HIP(d->net.forward(st, out->d, in->h, in->w));
     return HH_OK;
 }
 

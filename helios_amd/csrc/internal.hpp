@@ -131,6 +131,51 @@ hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const f
 // Host Hosek-Wilkie fit (sky_host.cpp)
 void sky_coefficients(float turbidity, float albedo, float normalized_sun_y, const float sun_dir[3], float out[30]);
 hipError_t launch_fill_f4(hipStream_t st, float4* p, float4 v, uint32_t n);
+
+// The learned denoiser (denoise.hip): U-Net of 14 3x3 convolutions + a 1x1
+// head, BF16 MFMA kernels with FP32 accumulation, Adam on FP32 master weights.
+constexpr int kDnLayers = 14;
+namespace dn {
+size_t net_num_params();
+void net_he_init(uint64_t seed, float* p);
+struct Net {
+    // parameters: FP32 master vector (layer by layer: W (Cout, 3, 3, Cin), b;
+    // then the head W (3, 32), b), gradients, Adam moments, BF16 copies
+    size_t nparams = 0;
+    float *master = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    size_t w_off[kDnLayers + 1] = {0};
+    __bf16* wf[kDnLayers] = {nullptr};   // forward layout
+    __bf16* wb[kDnLayers] = {nullptr};   // input-gradient layout
+    __bf16* whead = nullptr;
+    uint32_t steps = 0;
+    // activations for n images of H x W (multiples of 8)
+    uint32_t n = 0, H = 0, W = 0;
+    size_t cap_px = 0;
+    bool cap_train = false;
+    __bf16* xin = nullptr;
+    float* resid = nullptr;   // float4 per pixel: log1p radiance
+    float* out = nullptr;     // float4 per pixel: network output (log1p space)
+    float* dout = nullptr;
+    __bf16* act[kDnLayers] = {nullptr};
+    __bf16* pool[3] = {nullptr};
+    __bf16* gy[2] = {nullptr, nullptr};
+    float* gskip[3] = {nullptr};
+    float* gpool[3] = {nullptr};
+    float* loss = nullptr;
+
+    hipError_t init();
+    hipError_t zero_state(hipStream_t st);
+    hipError_t refresh_weights(hipStream_t st);
+    hipError_t ensure(uint32_t N, uint32_t Hp, uint32_t Wp, bool training);
+    hipError_t pack(hipStream_t st, const float* raw, const float4* rad, const float4* aux, uint32_t held, uint32_t h, uint32_t w);
+    hipError_t forward(hipStream_t st, float4* surface, uint32_t h, uint32_t w);
+    hipError_t loss_and_backward(hipStream_t st, const float* target, uint32_t h, uint32_t w);
+    hipError_t adam(hipStream_t st, float lr);
+    void release_acts();
+    void release();
+    ~Net() { release(); }
+};
+}  // namespace dn
 hipError_t launch_copy_probe(hipStream_t st, int num_cu, const float4* src, float4* dst, size_t n);
 
 }  // namespace hh

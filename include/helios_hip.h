@@ -52,6 +52,7 @@ typedef struct hh_context_t* hh_context;
 typedef struct hh_scene_t* hh_scene;
 typedef struct hh_surface_t* hh_surface;
 typedef struct hh_integrator_t* hh_integrator;
+typedef struct hh_denoiser_t* hh_denoiser;
 
 /* ---------------------------------------------------------------------------
  * Data layouts shared with the reference (byte compatible).
@@ -377,6 +378,49 @@ hh_status hh_context_reset_profile(hh_context ctx);
 /* Streams `bytes` through a plain copy kernel `iters` times and returns the
  * achieved read+write GB/s: the calibration point for HBM byte counters. */
 hh_status hh_bandwidth_probe(hh_context ctx, uint64_t bytes, uint32_t iters, double* out_gbps);
+
+/* ---------------------------------------------------------------------------
+ * Learned denoiser (no counterpart in the reference, which only accumulates;
+ * plain progressive accumulation stays the fallback and the judge of its
+ * error). A U-Net with levels of 32/48/64/96 channels: per level two 3x3
+ * convolutions + ReLU, 2x2 average pooling down, nearest upsampling and skip
+ * concatenation up, and a 1x1 head whose output is added to the input's
+ * log(1 + radiance). BF16 weights and activations on MFMA, FP32
+ * accumulation, FP32 master weights trained with Adam.
+ *
+ * Inputs per pixel (the network's 10 channels): log(1 + radiance),
+ * albedo, octahedral(normal), 1/(1+t), and the standard deviation of the mean
+ * luminance sqrt(M2 / (n (n-1))) from the surface's feature planes.
+ * Raw feature records (training and tests) hold 12 floats per pixel:
+ * radiance rgb, albedo rgb, 1/(1+t), normal xyz, M2, n (samples held).
+ * Images of any size are padded with zeros to multiples of 8.
+ *
+ * Parameters are one FP32 vector: per layer W (Cout, 3, 3, Cin) then b
+ * (Cout), for the 14 convolutions in order (enc0a, enc0b, enc1a, enc1b,
+ * enc2a, enc2b, mida, midb, dec2a, dec2b, dec1a, dec1b, dec0a, dec0b), then
+ * the head W (3, 32) and b (3): 480,675 floats.
+ * ------------------------------------------------------------------------- */
+hh_status hh_denoiser_create(hh_context ctx, uint64_t seed, hh_denoiser* out);  /* He-normal init */
+hh_status hh_denoiser_destroy(hh_denoiser dn);
+hh_status hh_denoiser_num_params(uint64_t* out);
+hh_status hh_denoiser_set_params(hh_denoiser dn, const float* params, uint64_t n);  /* resets Adam */
+hh_status hh_denoiser_get_params(hh_denoiser dn, float* params, uint64_t n);
+/* Gradient of the last hh_denoiser_loss_grad / train_step call. */
+hh_status hh_denoiser_get_grads(hh_denoiser dn, float* grads, uint64_t n);
+/* Denoises a surface with feature planes after `num_frames` progressive
+ * frames (the running mean then holds max(num_frames - 1, 1) samples) into
+ * `out` (same size, RGBA32F, alpha 1). */
+hh_status hh_denoise(hh_denoiser dn, hh_surface in, uint32_t num_frames, hh_surface out);
+/* Host-array entry points: raw records (n x h x w x 12) in, the network
+ * output in log(1 + radiance) space (n x h x w x 3) out. */
+hh_status hh_denoiser_forward(hh_denoiser dn, const float* feats, uint32_t n, uint32_t h, uint32_t w, float* out_log);
+/* Loss = mean over pixels and channels of (out - log(1 + target))^2 and its
+ * gradient (read with hh_denoiser_get_grads); target: n x h x w x 3. */
+hh_status hh_denoiser_loss_grad(hh_denoiser dn, const float* feats, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                                float* loss);
+/* loss_grad + one Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr. */
+hh_status hh_denoiser_train_step(hh_denoiser dn, const float* feats, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                                 float lr, float* loss);
 
 #ifdef __cplusplus
 }

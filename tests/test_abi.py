@@ -63,3 +63,13 @@ def test_argument_validation_without_device():
     st = abi.Settings(5, 5000, 0.0, 0, 0, 0)
     out = abi.LaunchConsts()
     assert L.hh_compute_launch_consts(C.byref(cam), 0, 10, 1, 0, C.byref(st), C.byref(out)) == 1
+
+
+def test_denoiser_parameter_count_without_device():
+    import ctypes as C
+    from helios_amd.runtime import lib
+    from oracle import unet
+    n = C.c_uint64()
+    lib().hh_denoiser_num_params.argtypes = [C.POINTER(C.c_uint64)]
+    assert lib().hh_denoiser_num_params(C.byref(n)) == 0
+    assert n.value == unet.num_params() == 480675
