@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-denoise", action="store_true")
+    p.add_argument("--ref-frames", type=int, default=4096,
+                   help="frames of the plain accumulation that judges both methods")
+    p.add_argument("--weights", default=str(ROOT / "helios_amd" / "data" / "denoiser_unet.bin"))
     p.add_argument("--probe", action="store_true",
                    help="counter-collection driver: 2 frames of the workload, then a 1 GiB copy kernel "
                         "(the calibration point for FETCH_SIZE / WRITE_SIZE)")
@@ -83,6 +87,87 @@ def cpu_baseline(desc, budget_s):
                 sample=f"{frames} full {w}x{h} progressive frame(s) (1 spp, 8 bounces) of the same scene and camera, "
                        f"{t:.1f} s; FP32 transcription with a CPU BVH2",
                 gsample_bounces_per_s=ext / t / 1e9)
+
+
+def relmse(x, r, eps=0.01):
+    import numpy as np
+    x = np.asarray(x[..., :3], np.float64)
+    r = np.asarray(r[..., :3], np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + eps)))
+
+
+def time_to_relmse(ctx, scene, desc, frame_ms, weights, ref_frames, target=0.01):
+    """Wall time for plain progressive accumulation and for the denoised
+    image to reach relMSE <= target against the plain ref_frames image.
+
+    One progressive run is checkpointed at ~10 % frame-count steps; at each
+    checkpoint the plain image and its denoised version are scored. Times are
+    frames x the measured frame time (+ the measured denoiser time)."""
+    import numpy as np
+    from helios_amd import runtime
+    from helios_amd.denoise import Denoiser
+    w, h = desc.width, desc.height
+    it = runtime.PathIntegrator(ctx)
+    it.set_max_ray_bounces(desc.max_ray_bounces)
+    it.set_max_samples(1 << 30)
+    ref_surf = runtime.Surface(ctx, w, h)
+    t0 = time.perf_counter()
+    it.render(scene, desc.camera, ref_surf, frames=ref_frames)
+    ref = ref_surf.read()
+    ref_s = time.perf_counter() - t0
+    del ref_surf
+
+    dn = Denoiser(ctx, seed=0)
+    trained = Path(weights).exists()
+    if trained:
+        dn.set_params(np.fromfile(weights, np.float32))
+    surf = runtime.Surface(ctx, w, h)
+    surf.enable_features()
+    out = runtime.Surface(ctx, w, h)
+    it.restart_bake()
+    # denoiser time: median of timed calls on a settled input
+    it.render(scene, desc.camera, surf, frames=2)
+    times = []
+    for _ in range(7):
+        ctx.sync()
+        t0 = time.perf_counter()
+        dn.denoise(surf, 2, out)
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+    dn_ms = float(np.median(times[2:]) * 1e3)
+
+    it.restart_bake()
+    done = 0
+    k = 2
+    curve = []
+    k_plain = k_dn = None
+    while k <= ref_frames // 2:
+        it.render(scene, desc.camera, surf, frames=k - done)
+        done = k
+        e_plain = relmse(surf.read(), ref)
+        dn.denoise(surf, k, out)
+        e_dn = relmse(out.read(), ref)
+        curve.append((k, round(e_plain, 5), round(e_dn, 5)))
+        if k_dn is None and e_dn <= target:
+            k_dn = k
+        if k_plain is None and e_plain <= target:
+            k_plain = k
+        if k_plain is not None and k_dn is not None:
+            break
+        k = max(k + 1, int(k * 1.1))
+    res = {
+        "target_relmse": target, "reference_frames": ref_frames, "reference_s": round(ref_s, 2),
+        "denoiser_weights": "trained" if trained else "untrained (no weights file)",
+        "denoise_ms": round(dn_ms, 3), "frame_ms": round(frame_ms, 4),
+        "plain_frames": k_plain, "denoised_frames": k_dn,
+        "plain_s": round(k_plain * frame_ms / 1e3, 4) if k_plain else None,
+        "denoised_s": round((k_dn * frame_ms + dn_ms) / 1e3, 4) if k_dn else None,
+        "curve_frames_plain_denoised": curve,
+    }
+    if k_plain and k_dn:
+        res["speedup"] = round(res["plain_s"] / res["denoised_s"], 2)
+    dn.close()
+    return res
 
 
 def main():
@@ -202,6 +287,9 @@ def main():
                                       if k != "counts" and v["launches"]}
 
     out["hbm_copy_gbps"] = round(ctx.bandwidth_probe(1 << 30, 4), 1)
+
+    if not a.no_denoise:
+        out["time_to_relmse_0.01"] = time_to_relmse(ctx, scene, desc, dt / a.steps * 1e3, a.weights, a.ref_frames)
 
     if not a.no_cpu_baseline:
         try:

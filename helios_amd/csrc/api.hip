@@ -1294,6 +1294,7 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
         v.spp = chunk;
         v.first_frame = it->num_acc;
         v.max_bounces = it->s.max_ray_bounces;
+        v.rr_start = it->s.rr_start_depth;
         v.shadow_bias = it->s.shadow_ray_bias;
         v.flags = it->s.flags;
         v.scene = 0;

@@ -142,6 +142,7 @@ struct ViewGPU {
     uint32_t spp;
     uint32_t first_frame;
     uint32_t max_bounces;
+    uint32_t rr_start;   // first depth at which Russian roulette may end a path
     uint32_t flags;
     uint32_t scene;      // index into the scene table
     uint32_t slot_base;  // first path slot of this view

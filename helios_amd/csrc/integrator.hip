@@ -474,9 +474,12 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
                     float ct = clampf(dot(p.normal, Wi), 0.0f, 1.0f);
                     f3 Tn = (Tp * (brdf * ct)) / pdf;
                     float prob = fmaxf(Tn.x, fmaxf(Tn.y, Tn.z));
+                    // the RR draw is consumed at every bounce (stream
+                    // alignment); before rr_start it neither ends nor rescales
                     float rr = next_float(rng);
-                    if (!(rr > prob)) {
-                        Tn = Tn * (1.0f / prob);
+                    const bool roulette = depth >= V.rr_start;
+                    if (!roulette || !(rr > prob)) {
+                        if (roulette) Tn = Tn * (1.0f / prob);
                         want_ext = true;
                         ext_o = p.pos;
                         ext_d = Wi;
@@ -659,8 +662,13 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
             float ct = clampf(dot(p.normal, Wi), 0.0f, 1.0f);
             f3 Tn = (T * (brdf * ct)) / pdf;
             float prob = fmaxf(Tn.x, fmaxf(Tn.y, Tn.z));
-            if (next_float(rng) > prob) break;
-            T = Tn * (1.0f / prob);
+            const float rr = next_float(rng);
+            if (depth >= V.rr_start) {
+                if (rr > prob) break;
+                T = Tn * (1.0f / prob);
+            } else {
+                T = Tn;
+            }
             o = p.pos;
             d = Wi;
             tmin = 0.0001f;

@@ -364,6 +364,10 @@ class PathIntegrator:
     def set_flags(self, flags):
         self._set(flags=flags)
 
+    def set_rr_start_depth(self, depth):
+        """First bounce at which Russian roulette may end a path (0 = the reference)."""
+        self._set(rr_start_depth=depth)
+
     def restart_bake(self):
         check(lib().hh_integrator_restart_bake(self.h))
 

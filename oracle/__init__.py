@@ -168,13 +168,14 @@ class OracleScene:
         return out.reshape(10, 3)
 
     def render(self, camera=None, width=None, height=None, frames=1, first_frame=0, max_ray_bounces=None,
-               crop=None, fp64=False, threads=None, flags=0, shadow_ray_bias=0.0, image=None):
+               crop=None, fp64=False, threads=None, flags=0, shadow_ray_bias=0.0, image=None, rr_start_depth=0):
         """Progressive samples first_frame..first_frame+frames-1 into an RGBA32F image."""
         d = self.desc
         width = width or d.width
         height = height or d.height
         cam = camera_struct(camera or d.camera)
-        st = settings_struct(max_ray_bounces or d.max_ray_bounces, flags=flags, shadow_ray_bias=shadow_ray_bias)
+        st = settings_struct(max_ray_bounces or d.max_ray_bounces, flags=flags, shadow_ray_bias=shadow_ray_bias,
+                             rr_start_depth=rr_start_depth)
         x0, y0, x1, y1 = crop or (0, 0, width, height)
         img = image if image is not None else np.zeros((height, width, 4), np.float32)
         stats = np.zeros(2, np.uint64)

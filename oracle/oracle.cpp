@@ -980,6 +980,7 @@ struct Tracer {
     uint32_t num_lights, max_bounces, flags;
     bool brute;
     Stats st;
+    uint32_t rr_start = 0;  // settings.rr_start_depth (0 = the reference: RR at every bounce)
 
     V3<R> env(V3<R> d) const { return sample_cube<R>(s.env, s.env_size, d); }
 
@@ -1103,8 +1104,13 @@ struct Tracer {
         R ct = gclamp(dot(p.normal, Wi), R(0), R(1));
         V3<R> Tn = (T * (brdf * ct)) / pdf;
         R prob = gmax(Tn.x, gmax(Tn.y, Tn.z));
-        if (R(next_float(rng)) > prob) return V3<R>(0, 0, 0);
-        Tn = Tn * (R(1) / prob);
+        // the RR draw is consumed at every bounce so the stream stays aligned;
+        // bounces before rr_start neither terminate nor rescale
+        const R u = R(next_float(rng));
+        if (depth >= rr_start) {
+            if (u > prob) return V3<R>(0, 0, 0);
+            Tn = Tn * (R(1) / prob);
+        }
         return trace_path(p.vertex.pos, Wi, R(0.0001), HH_TRACE_OPAQUE, Tn, depth + 1, rng);
     }
 };
@@ -1173,6 +1179,7 @@ static void render_impl(const Scene& s, const hh_camera& cam, const hh_settings&
     std::atomic<uint64_t> n_ext(0), n_sh(0);
     auto worker = [&]() {
         Tracer<R> tr{s, R(st.shadow_ray_bias), s.num_lights, st.max_ray_bounces, st.flags, false, {}};
+        tr.rr_start = st.rr_start_depth;
         for (;;) {
             uint32_t y = next_row.fetch_add(1);
             if (y >= y1) break;

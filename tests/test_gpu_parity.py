@@ -271,3 +271,21 @@ def test_render_batch_matches_single_views(gpu_ctx, cornell, small_field, cornel
     a2, _ = _render_gpu(gpu_ctx, g2, small_field, frames=2)
     assert np.array_equal(s1.read(), a1)
     assert np.array_equal(s2.read(), a2)
+
+
+def test_rr_start_depth_matches_oracle(gpu_ctx, cornell, cornell_pair):
+    """Russian roulette from bounce 3 (the small workload's setting): same
+    estimator on both sides, same RNG streams (the RR draw is always taken)."""
+    from helios_amd import runtime
+    g, o = cornell_pair
+    d = cornell
+    surf = runtime.Surface(gpu_ctx, d.width, d.height)
+    it = runtime.PathIntegrator(gpu_ctx)
+    it.set_max_ray_bounces(d.max_ray_bounces)
+    it.set_rr_start_depth(3)
+    it.render(g, d.camera, surf, frames=16)
+    ref = o.render(frames=16, rr_start_depth=3)
+    assert relmse(surf.read(), ref) < 1e-4
+    # and it is a different estimator from RR at every bounce
+    base = o.render(frames=16)
+    assert relmse(base, ref) > 1e-5
