@@ -96,6 +96,7 @@ struct hh_denoiser_t {
     float* d_feat = nullptr;     // staged raw records
     float* d_target = nullptr;   // staged targets
     size_t stage_px = 0;
+    uint32_t loss_kind = HH_DENOISER_LOSS_LOG_MSE;
     ~hh_denoiser_t() {
         if (d_feat) (void)hipFree(d_feat);
         if (d_target) (void)hipFree(d_target);
@@ -1378,35 +1379,166 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
         }
     for (auto& sg : scenes) HH_ARG(sg.bvh_width == scenes[0].bvh_width, "scenes of one batch must share the BVH width");
     unsigned long long zero[2] = {0, 0};
-    uint32_t total_px = 0;
-    for (uint32_t i = 0; i < n; i++) total_px += views[i].surface->w * views[i].surface->h;
-    HH_HIP(ctx->wf.ensure(total_px));
+    size_t total_px = 0;
+    for (uint32_t i = 0; i < n; i++) total_px += (size_t)views[i].surface->w * views[i].surface->h;
+    HH_ARG(total_px <= 0xFFFFFFFFull, "batch too large");
+    // Samples per wavefront pass: up to frames_in_flight per view, and no
+    // more path slots than queue_capacity (when set). Every view's samples
+    // are still accumulated in frame order.
+    uint32_t chunk = std::max<uint32_t>(1, ctx->tune.frames_in_flight);
+    if (ctx->tune.queue_capacity > 0)
+        chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, (uint32_t)(ctx->tune.queue_capacity / std::max<size_t>(total_px, 1))));
+    HH_HIP(ctx->wf.ensure((uint32_t)std::min<size_t>(total_px * std::min(chunk, max_spp), 0xFFFFFFFFull)));
     HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
-    for (uint32_t s = 0; s < max_spp; s++) {
+    for (uint32_t s = 0; s < max_spp; s += chunk) {
         std::vector<ViewGPU> vg;
-        uint32_t total = 0, maxb = 1;
+        size_t total = 0;
+        uint32_t maxb = 1;
         for (uint32_t i = 0; i < n; i++) {
             if (views[i].spp <= s) continue;
             const hh_surface sf = views[i].surface;
             ViewGPU v = make_view(views[i].camera, sf->w, sf->h);
-            v.spp = 1;
+            v.spp = std::min(chunk, views[i].spp - s);
             v.first_frame = views[i].first_frame + s;
             v.max_bounces = views[i].max_ray_bounces;
             v.shadow_bias = views[i].shadow_ray_bias;
             v.flags = 0;
             v.scene = scene_index[views[i].scene];
-            v.slot_base = total;
+            v.slot_base = (uint32_t)total;
             v.accum = sf->d;
             v.aux = sf->aux;
-            total += sf->w * sf->h;
+            total += (size_t)sf->w * sf->h * v.spp;
             maxb = std::max(maxb, v.max_bounces);
             vg.push_back(v);
         }
+        HH_ARG(total <= 0xFFFFFFFFull, "batch too large");
         std::string k = key;
         for (uint32_t i = 0; i < n; i++) k += "|" + std::to_string((uintptr_t)views[i].surface->d);
-        hh_status st = run_frame(ctx, scenes, vg, total, maxb, false, k);
+        hh_status st = run_frame(ctx, scenes, vg, (uint32_t)total, maxb, false, k);
         if (st != HH_OK) return st;
     }
+    return HH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner
+// ---------------------------------------------------------------------------
+// Milliseconds per progressive frame of `views` (rendered into the scratch
+// surfaces they name) under the context's current tuning.
+static hh_status time_views(hh_context ctx, const std::vector<hh_view>& views, uint32_t frames, double* ms) {
+    std::vector<hh_view> v = views;
+    for (auto& x : v) x.spp = std::max<uint32_t>(1, std::min<uint32_t>(frames, ctx->tune.frames_in_flight));
+    hh_status st = hh_render_batch(ctx, v.data(), (uint32_t)v.size());  // warm-up: graphs captured
+    if (st != HH_OK) return st;
+    for (auto& x : v) x.spp = frames;
+    hipEvent_t a, b;
+    HH_HIP(hipEventCreate(&a));
+    HH_HIP(hipEventCreate(&b));
+    HH_HIP(hipEventRecord(a, ctx->stream));
+    st = hh_render_batch(ctx, v.data(), (uint32_t)v.size());
+    hipError_t e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (st != HH_OK) return st;
+    if (e != hipSuccess) return hip_fail(e, "autotune timing");
+    *ms = (double)t / frames;
+    return HH_OK;
+}
+
+hh_status hh_context_autotune(hh_context ctx, const hh_view* views, uint32_t n, uint32_t frames, uint32_t flags,
+                              hh_tuning* out, double* out_ms) {
+    HH_ARG(ctx && views && n > 0, "bad argument");
+    HH_ARG(frames >= 1 && frames <= 256, "frames must be 1..256");
+    HH_HIP(hipSetDevice(ctx->device));
+    // scratch surfaces: the caller's images are not touched
+    std::vector<hh_surface> scratch;
+    std::vector<hh_view> v(views, views + n);
+    hh_status st = HH_OK;
+    for (uint32_t i = 0; i < n && st == HH_OK; i++) {
+        HH_ARG(views[i].surface && views[i].scene, "view without surface or scene");
+        hh_surface sf = nullptr;
+        st = hh_surface_create(ctx, views[i].surface->w, views[i].surface->h, &sf);
+        if (st == HH_OK) {
+            scratch.push_back(sf);
+            v[i].surface = sf;
+            v[i].first_frame = 0;
+        }
+    }
+    auto cleanup = [&]() {
+        for (hh_surface sf : scratch) hh_surface_destroy(sf);
+    };
+    if (st != HH_OK) {
+        cleanup();
+        return st;
+    }
+    Tuning best = ctx->tune;
+    double best_ms = 0.0;
+    auto trial = [&](const Tuning& t, double* ms) -> hh_status {
+        ctx->tune = t;
+        ctx->drop_graphs();
+        return time_views(ctx, v, frames, ms);
+    };
+    st = trial(best, &best_ms);
+    // coordinate descent, one knob at a time, twice over
+    for (int pass = 0; pass < 2 && st == HH_OK; pass++) {
+        const uint32_t fif[] = {1, 2, 4, 8, 16};
+        const int blk[] = {64, 128, 256};
+        const int bpc[] = {4, 8, 16, 32};
+        const uint32_t rf[] = {16, 32, 48, 64};
+        const int trav[] = {0, 1};
+        for (int knob = 0; knob < 5 && st == HH_OK; knob++) {
+            const int count = knob == 0 ? 5 : knob == 1 ? 3 : knob == 2 ? 4 : knob == 3 ? 4 : 2;
+            for (int c = 0; c < count && st == HH_OK; c++) {
+                Tuning t = best;
+                if (knob == 0) t.frames_in_flight = fif[c];
+                if (knob == 1) t.extend_block = t.shade_block = blk[c];
+                if (knob == 2) t.blocks_per_cu = bpc[c];
+                if (knob == 3) t.refill_lanes = rf[c];
+                if (knob == 4) t.traversal = trav[c];
+                if (t.frames_in_flight == best.frames_in_flight && t.extend_block == best.extend_block &&
+                    t.blocks_per_cu == best.blocks_per_cu && t.refill_lanes == best.refill_lanes && t.traversal == best.traversal)
+                    continue;
+                double ms = 0.0;
+                st = trial(t, &ms);
+                if (st == HH_OK && ms < best_ms) {
+                    best = t;
+                    best_ms = ms;
+                }
+            }
+        }
+    }
+    if (st == HH_OK && (flags & HH_AUTOTUNE_BVH_WIDTH)) {
+        // BVH width is a property of the committed scenes: re-commit them
+        std::vector<hh_scene> scenes;
+        for (uint32_t i = 0; i < n; i++)
+            if (std::find(scenes.begin(), scenes.end(), views[i].scene) == scenes.end()) scenes.push_back(views[i].scene);
+        const int cur = best.bvh_width;
+        Tuning t = best;
+        t.bvh_width = cur == 4 ? 8 : 4;
+        t.bvh_compressed = false;
+        ctx->tune = t;
+        for (hh_scene sc : scenes)
+            if (st == HH_OK) st = hh_scene_commit(sc);
+        double ms = 0.0;
+        if (st == HH_OK) st = trial(t, &ms);
+        if (st == HH_OK && ms < best_ms) {
+            best = t;
+            best_ms = ms;
+        } else if (st == HH_OK) {
+            ctx->tune = best;
+            for (hh_scene sc : scenes)
+                if (st == HH_OK) st = hh_scene_commit(sc);
+        }
+    }
+    ctx->tune = best;
+    ctx->drop_graphs();
+    cleanup();
+    if (st != HH_OK) return st;
+    if (out) hh_context_get_tuning(ctx, out);
+    if (out_ms) *out_ms = best_ms;
     return HH_OK;
 }
 
@@ -1602,11 +1734,18 @@ hh_status hh_denoiser_loss_grad(hh_denoiser d, const float* feats, const float* 
     if (s != HH_OK) return s;
     hipStream_t st = d->ctx->stream;
     HH_HIP(d->net.forward(st, nullptr, h, w));
-    HH_HIP(d->net.loss_and_backward(st, d->d_target, h, w));
+    HH_HIP(d->net.loss_and_backward(st, d->d_target, h, w, d->loss_kind));
     float l = 0.0f;
     HH_HIP(hipMemcpyAsync(&l, d->net.loss, 4, hipMemcpyDeviceToHost, st));
     HH_HIP(hipStreamSynchronize(st));
     if (loss) *loss = l / (3.0f * (float)n * (float)h * (float)w);
+    return HH_OK;
+}
+
+hh_status hh_denoiser_set_loss(hh_denoiser d, uint32_t kind) {
+    HH_ARG(d, "null denoiser");
+    HH_ARG(kind == HH_DENOISER_LOSS_LOG_MSE || kind == HH_DENOISER_LOSS_RELMSE, "unknown loss");
+    d->loss_kind = kind;
     return HH_OK;
 }
 

@@ -419,10 +419,24 @@ __global__ void k_head(const __bf16* __restrict__ act, const __bf16* __restrict_
     }
 }
 
-// MSE in log1p space over the valid pixels: dout = 2 (out - log1p(t)) / count
-// (0 on padding); the loss sum goes to *loss.
+// Loss over the valid pixels (0 gradient on padding); the sum goes to *loss.
+//   kind 0: MSE in log1p space, dout = 2 (out - log1p(t)) / count
+//   kind 1: relMSE of y = expm1(out) against t, (y - t)^2 / (t^2 + 0.01),
+//           dout = 2 (y - t) (y + 1) / ((t^2 + 0.01) count)
+__device__ inline float loss_term(float o, float t, uint32_t kind, float inv, float& g) {
+    t = fmaxf(t, 0.0f);
+    if (kind == 0u) {
+        const float d = o - log1pf(t);
+        g = d * inv;
+        return d * d;
+    }
+    const float y = expm1f(o), e = y - t, wgt = 1.0f / (t * t + 0.01f);
+    g = e * wgt * (y + 1.0f) * inv;
+    return e * e * wgt;
+}
+
 __global__ void k_loss(const float4* __restrict__ out, const float* __restrict__ target, uint32_t N, uint32_t h, uint32_t w,
-                       uint32_t Hp, uint32_t Wp, float4* __restrict__ dout, float* __restrict__ loss) {
+                       uint32_t Hp, uint32_t Wp, uint32_t kind, float4* __restrict__ dout, float* __restrict__ loss) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.0f;
     if (i < (size_t)N * Hp * Wp) {
@@ -434,11 +448,10 @@ __global__ void k_loss(const float4* __restrict__ out, const float* __restrict__
             const size_t src = ((size_t)n * h + y) * w + x;
             const float inv = 2.0f / (3.0f * (float)N * (float)h * (float)w);
             const float4 o = out[i];
-            const float d0 = o.x - log1pf(fmaxf(target[src * 3 + 0], 0.0f));
-            const float d1 = o.y - log1pf(fmaxf(target[src * 3 + 1], 0.0f));
-            const float d2 = o.z - log1pf(fmaxf(target[src * 3 + 2], 0.0f));
-            l = d0 * d0 + d1 * d1 + d2 * d2;
-            d = make_float4(d0 * inv, d1 * inv, d2 * inv, 0.0f);
+            float g0, g1, g2;
+            l = loss_term(o.x, target[src * 3 + 0], kind, inv, g0) + loss_term(o.y, target[src * 3 + 1], kind, inv, g1) +
+                loss_term(o.z, target[src * 3 + 2], kind, inv, g2);
+            d = make_float4(g0, g1, g2, 0.0f);
         }
         dout[i] = d;
     }
@@ -580,16 +593,18 @@ size_t net_num_params() {
     return n + 3 * 32 + 3;
 }
 
-// He-normal weights (standard deviation sqrt(2 / fan_in)), zero biases;
-// Box-Muller over a 64-bit LCG, so the vector depends only on the seed.
+// He-normal weights (standard deviation sqrt(2 / fan_in)), zero biases, and
+// a head at a tenth of that scale, so an untrained network returns nearly its
+// noisy input (the head adds to the input's log radiance); Box-Muller over a
+// 64-bit LCG, so the vector depends only on the seed.
 void net_he_init(uint64_t seed, float* p) {
     uint64_t s = seed * 6364136223846793005ull + 1442695040888963407ull;
     auto uni = [&]() {
         s = s * 6364136223846793005ull + 1442695040888963407ull;
         return ((double)(s >> 11) + 0.5) * (1.0 / 9007199254740992.0);
     };
-    auto fill = [&](float* w, size_t nw, size_t fan_in) {
-        const double sd = std::sqrt(2.0 / (double)fan_in);
+    auto fill = [&](float* w, size_t nw, size_t fan_in, double scale = 1.0) {
+        const double sd = scale * std::sqrt(2.0 / (double)fan_in);
         for (size_t i = 0; i < nw; i++) {
             const double u1 = uni(), u2 = uni();
             w[i] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
@@ -602,7 +617,7 @@ void net_he_init(uint64_t seed, float* p) {
         for (int i = 0; i < L.cout; i++) p[off + nw + i] = 0.0f;
         off += nw + L.cout;
     }
-    fill(p + off, 96, 32);
+    fill(p + off, 96, 32, 0.1);
     for (int i = 0; i < 3; i++) p[off + 96 + i] = 0.0f;
 }
 
@@ -766,14 +781,14 @@ hipError_t Net::forward(hipStream_t st, float4* surface, uint32_t h, uint32_t w)
     return hipGetLastError();
 }
 
-hipError_t Net::loss_and_backward(hipStream_t st, const float* target, uint32_t h, uint32_t w) {
+hipError_t Net::loss_and_backward(hipStream_t st, const float* target, uint32_t h, uint32_t w, uint32_t kind) {
     const uint32_t N = n;
     const size_t P = (size_t)N * H * W;
     auto Hl = [&](int lv) { return H >> lv; };
     auto Wl = [&](int lv) { return W >> lv; };
     k_zero_f32<<<1, 64, 0, st>>>(loss, 1);
     k_zero_f32<<<cdiv(nparams, 256), 256, 0, st>>>(grad, nparams);
-    k_loss<<<cdiv(P, 256), 256, 0, st>>>((const float4*)out, target, N, h, w, H, W, (float4*)dout, loss);
+    k_loss<<<cdiv(P, 256), 256, 0, st>>>((const float4*)out, target, N, h, w, H, W, kind, (float4*)dout, loss);
     float* gw[kDnLayers];
     float* gb[kDnLayers];
     for (int l = 0; l < kDnLayers; l++) {

@@ -169,7 +169,7 @@ struct Net {
     hipError_t ensure(uint32_t N, uint32_t Hp, uint32_t Wp, bool training);
     hipError_t pack(hipStream_t st, const float* raw, const float4* rad, const float4* aux, uint32_t held, uint32_t h, uint32_t w);
     hipError_t forward(hipStream_t st, float4* surface, uint32_t h, uint32_t w);
-    hipError_t loss_and_backward(hipStream_t st, const float* target, uint32_t h, uint32_t w);
+    hipError_t loss_and_backward(hipStream_t st, const float* target, uint32_t h, uint32_t w, uint32_t kind);
     hipError_t adam(hipStream_t st, float lr);
     void release_acts();
     void release();

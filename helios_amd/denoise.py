@@ -26,6 +26,7 @@ _SIGS = {
     "hh_denoiser_forward": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_denoiser_loss_grad": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                               C.POINTER(C.c_float)],
+    "hh_denoiser_set_loss": [C.c_void_p, C.c_uint32],
     "hh_denoiser_train_step": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
                                C.POINTER(C.c_float)],
 }
@@ -69,6 +70,10 @@ class Denoiser:
             self.close()
         except Exception:
             pass
+
+    def set_loss(self, kind):
+        """"log_mse" (default) or "relmse"."""
+        check(_lib().hh_denoiser_set_loss(self.h, {"log_mse": 0, "relmse": 1}[kind]))
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32).ravel()

@@ -128,6 +128,7 @@ def main(argv=None):
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--batch", type=int, default=8)
     p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--loss", default="relmse", choices=["relmse", "log_mse"])
     p.add_argument("--init", default=None, help="start from these parameters (float32 file)")
     a = p.parse_args(argv)
     ctx = runtime.Context(0)
@@ -136,6 +137,7 @@ def main(argv=None):
                              a.crop)
     print(json.dumps({"dataset": int(len(X)), "targets": int(len(T))}))
     dn = Denoiser(ctx, seed=0)
+    dn.set_loss(a.loss)
     if a.init:
         dn.set_params(np.fromfile(a.init, np.float32))
     losses = train(dn, X, T, TI, a.steps, a.batch, a.lr)

@@ -77,6 +77,8 @@ _SIGS = {
     "hh_compute_launch_consts": [C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.POINTER(abi.Settings), C.POINTER(abi.LaunchConsts)],
     "hh_render_batch": [C.c_void_p, C.c_void_p, C.c_uint32],
+    "hh_context_autotune": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(abi.Tuning),
+                            C.POINTER(C.c_double)],
     "hh_trace_rays": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_primary_hits": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_sky_fit": [C.POINTER(abi.SkyParams), C.c_void_p, C.c_void_p],
@@ -411,6 +413,28 @@ def render_batch(ctx, views):
         arr[i].shadow_ray_bias = v.get("shadow_ray_bias", 0.0)
         arr[i].first_frame = v.get("first_frame", 0)
     check(lib().hh_render_batch(ctx.h, arr, len(views)))
+
+
+def _view_array(views):
+    arr = (abi.View * len(views))()
+    for i, v in enumerate(views):
+        arr[i].scene = v["scene"].h.value
+        arr[i].surface = v["surface"].h.value
+        arr[i].camera = camera_struct(v["camera"])
+        arr[i].spp = v.get("spp", 1)
+        arr[i].max_ray_bounces = v.get("max_ray_bounces", 8)
+        arr[i].shadow_ray_bias = v.get("shadow_ray_bias", 0.0)
+        arr[i].first_frame = v.get("first_frame", 0)
+    return arr
+
+
+def autotune(ctx, views, frames=8, bvh_width=False):
+    """hh_context_autotune: keeps the fastest tuning on ctx; returns (tuning, ms per frame)."""
+    arr = _view_array(views)
+    t = abi.Tuning()
+    ms = C.c_double()
+    check(lib().hh_context_autotune(ctx.h, arr, len(views), frames, 1 if bvh_width else 0, C.byref(t), C.byref(ms)))
+    return ctx.tuning(), ms.value
 
 
 def launch_consts(camera, width, height, num_lights=1, frame=0, max_ray_bounces=5):

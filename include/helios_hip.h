@@ -322,7 +322,11 @@ typedef struct hh_view {
     float shadow_ray_bias;
     uint32_t first_frame;      /* num_frames of the first sample */
 } hh_view;
+/* Renders spp progressive samples of every view, up to frames_in_flight
+ * samples per view per wavefront pass (and at most queue_capacity path slots
+ * per pass when that is set); each view accumulates in frame order. */
 hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t num_views);
+
 
 /* ---------------------------------------------------------------------------
  * Traversal entry points (closest / any hit with the reference's ray-flag
@@ -362,6 +366,20 @@ typedef struct hh_tuning {
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);
+/* Autotuner (the reference leaves all of this to its driver): times
+ * `frames` progressive frames of the views, rendered into scratch surfaces
+ * (the views' own surfaces are untouched), under candidate tunings and keeps
+ * the fastest on the context: coordinate descent, twice over, on
+ * frames_in_flight {1,2,4,8,16}, extend_block {64,128,256},
+ * persistent_blocks_per_cu {4,8,16,32}, refill_lanes {16,32,48,64} and
+ * traversal {0,1}; with HH_AUTOTUNE_BVH_WIDTH it also re-commits the views'
+ * scenes with the other BVH width and keeps the faster (a re-commit restarts
+ * the progressive accumulation of integrators on those scenes). Tuning never
+ * changes an image (tested bit-identical). Returns the chosen tuning and its
+ * milliseconds per frame. */
+#define HH_AUTOTUNE_BVH_WIDTH 0x1u
+hh_status hh_context_autotune(hh_context ctx, const hh_view* views, uint32_t num_views, uint32_t frames, uint32_t flags,
+                              hh_tuning* out, double* out_ms_per_frame);
 
 /* ---------------------------------------------------------------------------
  * Measurement (replaces the reference's scoped GPU timestamps,
@@ -414,8 +432,14 @@ hh_status hh_denoise(hh_denoiser dn, hh_surface in, uint32_t num_frames, hh_surf
 /* Host-array entry points: raw records (n x h x w x 12) in, the network
  * output in log(1 + radiance) space (n x h x w x 3) out. */
 hh_status hh_denoiser_forward(hh_denoiser dn, const float* feats, uint32_t n, uint32_t h, uint32_t w, float* out_log);
-/* Loss = mean over pixels and channels of (out - log(1 + target))^2 and its
- * gradient (read with hh_denoiser_get_grads); target: n x h x w x 3. */
+/* Training loss, mean over pixels and channels (target: n x h x w x 3):
+ *   HH_DENOISER_LOSS_LOG_MSE (default)  (out - log(1 + target))^2
+ *   HH_DENOISER_LOSS_RELMSE             (y - t)^2 / (t^2 + 0.01), y = exp(out) - 1
+ * hh_denoiser_loss_grad returns the loss and keeps its gradient (read with
+ * hh_denoiser_get_grads). */
+#define HH_DENOISER_LOSS_LOG_MSE 0u
+#define HH_DENOISER_LOSS_RELMSE 1u
+hh_status hh_denoiser_set_loss(hh_denoiser dn, uint32_t kind);
 hh_status hh_denoiser_loss_grad(hh_denoiser dn, const float* feats, const float* target, uint32_t n, uint32_t h, uint32_t w,
                                 float* loss);
 /* loss_grad + one Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr. */

@@ -233,6 +233,16 @@ def loss_mse(out, target_log):
     return float(np.mean(d * d)), 2.0 * d / d.size
 
 
+def loss_relmse(out, target, eps=0.01):
+    """relMSE of the linear prediction y = expm1(out) against a linear target:
+    mean of (y - t)^2 / (t^2 + eps); gradient with respect to out."""
+    y = np.expm1(out)
+    t = np.maximum(np.asarray(target, np.float64), 0.0)
+    wgt = 1.0 / (t * t + eps)
+    e = y - t
+    return float(np.mean(e * e * wgt)), 2.0 * e * wgt * (y + 1.0) / e.size
+
+
 def backward(params, cache, dout):
     """Gradients of a scalar loss w.r.t. every parameter given dL/dout."""
     q = bf16 if cache["emulate"] else (lambda a: a)

@@ -159,3 +159,20 @@ def test_forward_shapes_and_residual(emulate):
     out, _ = unet.forward(params, x, emulate_bf16=emulate)
     assert out.shape == (2, 16, 8, 3)
     np.testing.assert_allclose(out, x[..., :3], rtol=1e-6)
+
+
+def test_relmse_loss_gradient_finite_differences():
+    rng = np.random.default_rng(9)
+    out = rng.normal(0.0, 0.5, size=(1, 2, 3, 3))
+    t = rng.gamma(1.0, 0.3, size=(1, 2, 3, 3))
+    _, g = unet.loss_relmse(out, t)
+    eps = 1e-6
+    for i in range(out.size):
+        o1, o2 = out.copy().ravel(), out.copy().ravel()
+        o1[i] += eps
+        o2[i] -= eps
+        fd = (unet.loss_relmse(o1.reshape(out.shape), t)[0] - unet.loss_relmse(o2.reshape(out.shape), t)[0]) / (2 * eps)
+        assert abs(fd - g.ravel()[i]) <= 1e-7 + 1e-5 * abs(fd)
+    # the value is the bench's relMSE of the linear prediction
+    y = np.expm1(out)
+    assert np.isclose(unet.loss_relmse(out, t)[0], np.mean((y - t) ** 2 / (t * t + 0.01)))

@@ -84,16 +84,22 @@ def test_forward_padding(dn):
     assert rel(got, ref[:, :13, :21]) <= 2e-3
 
 
-def test_gradients_match_oracle(dn):
+@pytest.mark.parametrize("kind", ["log_mse", "relmse"])
+def test_gradients_match_oracle(dn, kind):
     rng = np.random.default_rng(12)
     params = perturbed_params(5)
     dn.set_params(unet.flatten(params).astype(np.float32))
+    dn.set_loss(kind)
     feats = random_feats(rng, 2, 16, 16)
     target = rng.gamma(1.0, 0.4, size=(2, 16, 16, 3)).astype(np.float32)
     loss = dn.loss_grad(feats, target)
+    dn.set_loss("log_mse")
     g = unet.unflatten(dn.grads())
     out, cache = unet.forward(params, unet.pack_inputs(feats), emulate_bf16=True)
-    ref_loss, dout = unet.loss_mse(out, np.log1p(target.astype(np.float64)))
+    if kind == "log_mse":
+        ref_loss, dout = unet.loss_mse(out, np.log1p(target.astype(np.float64)))
+    else:
+        ref_loss, dout = unet.loss_relmse(out, target.astype(np.float64))
     assert abs(loss - ref_loss) <= 1e-3 * ref_loss
     ref = unet.backward(params, cache, dout)
     for name, _, _ in unet.param_shapes():
@@ -111,7 +117,7 @@ def test_training_lowers_loss(gpu_ctx):
     clean = np.broadcast_to(rng.uniform(0.1, 0.8, size=(n, 1, 1, 3)), (n, h, w, 3)).astype(np.float32)
     feats = random_feats(rng, n, h, w)
     feats[..., 0:3] = np.maximum(clean + rng.normal(0.0, 0.15, size=clean.shape), 0.0)
-    d = Denoiser(gpu_ctx, seed=2)
+    d = Denoiser(gpu_ctx, seed=2, params=unet.flatten(unet.he_init(2)).astype(np.float32))
     first = d.train_step(feats, clean, 1e-3)
     for _ in range(59):
         last = d.train_step(feats, clean, 1e-3)
