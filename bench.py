@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--ref-frames", type=int, default=4096,
                    help="frames of the plain accumulation that judges both methods")
     p.add_argument("--weights", default=str(ROOT / "helios_amd" / "data" / "denoiser_unet.bin"))
+    p.add_argument("--denoise-probe", action="store_true",
+                   help="counter-collection driver: 3 frames with feature planes, then 10 denoiser passes")
     p.add_argument("--probe", action="store_true",
                    help="counter-collection driver: 2 frames of the workload, then a 1 GiB copy kernel "
                         "(the calibration point for FETCH_SIZE / WRITE_SIZE)")
@@ -155,10 +157,15 @@ def time_to_relmse(ctx, scene, desc, frame_ms, weights, ref_frames, target=0.01)
         if k_plain is not None and k_dn is not None:
             break
         k = max(k + 1, int(k * 1.1))
+    from helios_amd.denoise import forward_flops
+    flops = forward_flops(h, w)
     res = {
         "target_relmse": target, "reference_frames": ref_frames, "reference_s": round(ref_s, 2),
         "denoiser_weights": "trained" if trained else "untrained (no weights file)",
         "denoise_ms": round(dn_ms, 3), "frame_ms": round(frame_ms, 4),
+        "denoiser_forward": {"gflop": round(flops / 1e9, 2), "tflops": round(flops / (dn_ms * 1e-3) / 1e12, 1),
+                             "frac_of_bf16_mfma_peak": round(flops / (dn_ms * 1e-3) / 2.5e15, 4),
+                             "timing": "host clock around one synchronized hh_denoise call (median of 5)"},
         "plain_frames": k_plain, "denoised_frames": k_dn,
         "plain_s": round(k_plain * frame_ms / 1e3, 4) if k_plain else None,
         "denoised_s": round((k_dn * frame_ms + dn_ms) / 1e3, 4) if k_dn else None,
@@ -194,6 +201,22 @@ def main():
     if a.megakernel:
         from helios_amd.abi import HH_FLAG_MEGAKERNEL
         it.set_flags(HH_FLAG_MEGAKERNEL)
+
+    if a.denoise_probe:
+        from helios_amd.denoise import Denoiser, forward_flops
+        import numpy as np
+        surf.enable_features()
+        it.render(scene, desc.camera, surf, frames=3)
+        dn = Denoiser(ctx, seed=0)
+        if Path(a.weights).exists():
+            dn.set_params(np.fromfile(a.weights, np.float32))
+        out_s = runtime.Surface(ctx, a.width, a.height)
+        for _ in range(10):
+            dn.denoise(surf, 3, out_s)
+        ctx.sync()
+        print(json.dumps({"probe": "10 denoiser passes at %dx%d" % (a.width, a.height),
+                          "gflop_per_pass": forward_flops(a.height, a.width) / 1e9}))
+        return 0
 
     if a.probe:
         it.render(scene, desc.camera, surf, frames=a.frames_in_flight)

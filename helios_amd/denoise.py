@@ -15,6 +15,18 @@ from .runtime import check, lib
 
 N_FEAT = 12
 
+# (input channels, output channels, pyramid level) of the 14 convolutions
+CONVS = [(10, 32, 0), (32, 32, 0), (32, 48, 1), (48, 48, 1), (48, 64, 2), (64, 64, 2), (64, 96, 3), (96, 96, 3),
+         (160, 64, 2), (64, 64, 2), (112, 48, 1), (48, 48, 1), (80, 32, 0), (32, 32, 0)]
+
+
+def forward_flops(h, w):
+    """Algorithmic FLOPs (2 per multiply-add) of one forward pass on an h x w
+    image padded to multiples of 8: 84,624 multiply-adds per pixel."""
+    hp, wp = (h + 7) // 8 * 8, (w + 7) // 8 * 8
+    macs = sum(9 * ci * co * (hp >> lv) * (wp >> lv) for ci, co, lv in CONVS) + 32 * 3 * hp * wp
+    return 2 * macs
+
 _SIGS = {
     "hh_denoiser_create": [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)],
     "hh_denoiser_destroy": [C.c_void_p],

@@ -3,6 +3,8 @@
   python -m helios_amd.profiling stats  <kernel_stats.csv> --cmd "<command>" --out profiles/<name>.md
   python -m helios_amd.profiling traffic <fetch counter_collection.csv> <write counter_collection.csv>
          --kernel k_extend --cmd "<command>" --out profiles/extend_traffic.json
+  python -m helios_amd.profiling mfma <kernel_stats.csv> <counter_collection.csv> --cmd "<commands>"
+         --out profiles/<name>.md
 
 `traffic` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) by what the same
 counters read for the plain copy kernel k_copy_probe, which moves a known
@@ -17,7 +19,24 @@ from pathlib import Path
 PROBE_BYTES = 1 << 30
 
 
+def demangle(name):
+    """Readable form of the denoiser's mangled template kernel names
+    (_ZN2hh2dn9k_conv3x3ILi48ELi2ELi32ELi32ELi0EE... -> hh::dn::k_conv3x3<48, 2, 32, 32, 0>)."""
+    import re
+    m = re.match(r"_ZN2hh2dn(\d+)", name)
+    if not m:
+        return name
+    n = int(m.group(1))
+    rest = name[m.end():]
+    kname, tail = rest[:n], rest[n:]
+    args = re.match(r"I((?:Li-?\d+E)+)E", tail)
+    if not args:
+        return "hh::dn::" + kname
+    return f"hh::dn::{kname}<{', '.join(re.findall(r'Li(-?[0-9]+)E', args.group(1)))}>"
+
+
 def short(name):
+    name = demangle(name)
     n = name.split("(")[0].replace("void ", "")
     return n if len(n) < 90 else n[:87] + "..."
 
@@ -38,6 +57,41 @@ def per_dispatch(csv_path):
     for r in csv.DictReader(open(csv_path)):
         agg[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
     return agg
+
+
+def mfma_md(stats_csv, pmc_csv, cmd, peak_tflops=2500.0, simds=1024):
+    """Per denoiser kernel: average duration (kernel trace), BF16 MFMA FLOPs
+    per dispatch (SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512), the FLOP rate they
+    imply, and MFMA busy fraction (SQ_VALU_MFMA_BUSY_CYCLES over
+    GRBM_GUI_ACTIVE per XCD x SIMDs)."""
+    dur = {}
+    for r in csv.DictReader(open(stats_csv)):
+        dur[short(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]))
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    xcds = 1
+    for r in csv.DictReader(open(pmc_csv)):
+        agg[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    lines = [f"Commands: {cmd}", "", "| kernel | calls | avg us | MFMA GFLOP / dispatch | TFLOP/s | MFMA busy % |",
+             "|---|---|---|---|---|---|"]
+    tot_ns = tot_flop = 0.0
+    for k, (calls, ns) in sorted(dur.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        if "hh::dn::" not in k:
+            continue
+        c = agg.get(k, {})
+        mops = sum(c.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", [0])) / max(len(c.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", [1])), 1)
+        busy = sum(c.get("SQ_VALU_MFMA_BUSY_CYCLES", [0])) / max(len(c.get("SQ_VALU_MFMA_BUSY_CYCLES", [1])), 1)
+        gui = sum(c.get("GRBM_GUI_ACTIVE", [0])) / max(len(c.get("GRBM_GUI_ACTIVE", [1])), 1)
+        flop = mops * 512
+        # GRBM_GUI_ACTIVE is summed over the 8 XCDs
+        util = 100.0 * busy / (gui / 8 * simds) if gui else 0.0
+        lines.append(f"| {k} | {calls} | {ns / 1e3:.1f} | {flop / 1e9:.2f} | {flop / ns / 1e3:.1f} | {util:.1f} |")
+        if k not in ("hh::dn::k_relayout", "hh::dn::k_zero_f32", "hh::dn::k_to_bf16"):
+            tot_ns += ns * calls
+            tot_flop += flop * calls
+    lines += ["", f"Forward passes: {tot_flop / 1e9:.1f} GFLOP of MFMA work in {tot_ns / 1e6:.3f} ms of kernel time "
+              f"= {tot_flop / tot_ns / 1e3:.1f} TFLOP/s = {tot_flop / tot_ns / 1e3 / peak_tflops:.3f} of the "
+              f"{peak_tflops:.0f} TFLOP/s dense BF16 peak."]
+    return "\n".join(lines) + "\n"
 
 
 def traffic(fetch_csv, write_csv, kernel, cmd):
@@ -76,10 +130,17 @@ def main():
     t.add_argument("--kernel", default="k_extend")
     t.add_argument("--cmd", required=True)
     t.add_argument("--out", required=True)
+    m = sub.add_parser("mfma")
+    m.add_argument("stats_csv")
+    m.add_argument("pmc_csv")
+    m.add_argument("--cmd", required=True)
+    m.add_argument("--out", required=True)
     a = p.parse_args()
     if a.what == "stats":
         text = (f"# {a.title}\n\n" if a.title else "") + stats_md(a.csv, a.cmd)
         Path(a.out).write_text(text)
+    elif a.what == "mfma":
+        Path(a.out).write_text(mfma_md(a.stats_csv, a.pmc_csv, a.cmd))
     else:
         Path(a.out).write_text(json.dumps(traffic(a.fetch_csv, a.write_csv, a.kernel, a.cmd), indent=1) + "\n")
     print("wrote", a.out)

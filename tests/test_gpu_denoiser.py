@@ -15,6 +15,7 @@ accumulation order and the rare BF16 rounding flip it causes. Bars:
 import numpy as np
 import pytest
 
+from helios_amd import scenegen
 from oracle import unet
 
 pytestmark = pytest.mark.gpu
@@ -144,3 +145,43 @@ def test_denoise_surface_equals_host_path(gpu_ctx, dn):
     ref = np.maximum(np.expm1(dn.forward(rec)[0].astype(np.float64)), 0.0)
     np.testing.assert_allclose(got[..., :3], ref, rtol=1e-5, atol=1e-6)
     assert np.all(got[..., 3] == 1.0)
+
+
+def test_trained_network_gain_on_held_out_view(gpu_ctx):
+    """What the learned part buys: on a view of the held-out benchmark scene
+    (seed 0; the network was trained on seeds 1-16), after 3 frames (2
+    samples), the trained network's relMSE against a 512-frame plain image is
+    far below both the plain image's and the untrained network's."""
+    from pathlib import Path
+    from helios_amd import runtime
+    from helios_amd.denoise import Denoiser
+    weights = Path(__file__).resolve().parents[1] / "helios_amd" / "data" / "denoiser_unet.bin"
+    d = scenegen.icosphere_field(seed=0, width=960, height=540, max_ray_bounces=8)
+    scene = runtime.Scene(gpu_ctx, d)
+    it = runtime.PathIntegrator(gpu_ctx)
+    it.set_max_ray_bounces(8)
+    it.set_max_samples(1 << 20)
+    ref_s = runtime.Surface(gpu_ctx, d.width, d.height)
+    it.render(scene, d.camera, ref_s, frames=512)
+    ref = ref_s.read()
+    it.restart_bake()
+    surf = runtime.Surface(gpu_ctx, d.width, d.height)
+    surf.enable_features()
+    it.render(scene, d.camera, surf, frames=3)
+    out = runtime.Surface(gpu_ctx, d.width, d.height)
+
+    def err(img):
+        x, r = img[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+        return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+    plain = err(surf.read())
+    untrained = Denoiser(gpu_ctx, seed=0)
+    untrained.denoise(surf, 3, out)
+    e_untrained = err(out.read())
+    untrained.close()
+    trained = Denoiser(gpu_ctx, params=np.fromfile(weights, np.float32))
+    trained.denoise(surf, 3, out)
+    e_trained = err(out.read())
+    trained.close()
+    assert e_trained < 0.25 * plain, (e_trained, plain)
+    assert e_trained < 0.25 * e_untrained, (e_trained, e_untrained)
