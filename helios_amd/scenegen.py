@@ -424,3 +424,111 @@ def random_rays(n, lo, hi, seed=0, tmin=0.0, tmax=1e4):
     r["tmin"] = tmin
     r["tmax"] = tmax
     return r
+
+
+def field_for_triangles(seed, target_tris, width, height, max_ray_bounces=8, spp=1):
+    """An icosphere field of about target_tris triangles: the sphere level and
+    count are chosen from the target (levels 2..5, at most 120 spheres), plus
+    ground and 1 % of the budget in leaf quads."""
+    leaves = int(min(2000, max(0, target_tris // 100)))
+    body = max(1, target_tris - 2 * leaves - 2)
+    level = 2
+    while level < 5 and body / (20 * 4 ** level) > 40:
+        level += 1
+    n = int(max(1, min(120, round(body / (20 * 4 ** level)))))
+    return icosphere_field(seed=seed, n_spheres=n, level=level, n_leaves=leaves, width=width, height=height,
+                           max_ray_bounces=max_ray_bounces, spp=spp)
+
+
+def batched_workload(n=64, seed0=0, min_side=128, max_side=1024, min_tris=10_000, max_tris=2_000_000,
+                     max_ray_bounces=8, spp=16):
+    """The batched workload: n independent scenes and cameras (seeds seed0..),
+    square resolutions log-uniform in [min_side, max_side] and triangle counts
+    log-uniform in [min_tris, max_tris], built with the icosphere-field
+    generator."""
+    rng = np.random.Generator(np.random.PCG64(seed0 + 7919))
+    out = []
+    for i in range(n):
+        side = int(round(math.exp(rng.uniform(math.log(min_side), math.log(max_side)))))
+        tris = int(round(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))))
+        d = field_for_triangles(seed0 + i, tris, side, side, max_ray_bounces, spp)
+        ang = rng.uniform(0, 2 * np.pi)
+        dist = rng.uniform(60.0, 150.0)
+        d.camera = look_at_camera((dist * math.cos(ang), rng.uniform(10.0, 50.0), dist * math.sin(ang)),
+                                  (0.0, 0.0, 0.0), fov_y=60.0, near=1.0, far=1000.0, aperture=0.0)
+        out.append(d)
+    return out
+
+
+def _grid_panel(n, size):
+    """n x n quad grid (2 n^2 triangles) in the xz plane, facing -y, centred at 0."""
+    s = np.linspace(-size / 2, size / 2, n + 1)
+    X, Z = np.meshgrid(s, s, indexing="ij")
+    P = np.stack([X.ravel(), np.zeros(X.size), Z.ravel()], 1)
+    N = np.tile([0.0, -1.0, 0.0], (P.shape[0], 1))
+    U = np.stack([(X.ravel() / size) + 0.5, (Z.ravel() / size) + 0.5], 1)
+    idx = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            idx += [a, c, b, a, d, c]          # wound to face -y
+    return P, N, U, np.array(idx, np.uint32)
+
+
+def combined_workload(seed=0, width=3840, height=2160, n_meshes=40, n_instances=1000, max_ray_bounces=16, spp=2,
+                      levels=(4, 5), level5_fraction=0.35, emissive_triangles=1024):
+    """The combined 4K workload: n_instances instances of n_meshes seeded
+    displaced meshes (levels 4 and 5: ~10 M instanced triangles) in one TLAS
+    on a 400 x 400 ground, emissive panels with emissive_triangles triangles in
+    total (area lights), the Hosek-Wilkie sky with its sun and 16 bounces.
+    Returns (SceneDesc, [4 cameras])."""
+    rng = np.random.Generator(np.random.PCG64(seed + 104729))
+    d = SceneDesc(width=width, height=height, spp=spp, max_ray_bounces=max_ray_bounces)
+    d.sky = (3.0, 0.1, 1.15, 512)
+    half = 200.0
+    gm = len(d.materials)
+    d.materials.append(material(linear_albedo((0.4, 0.4, 0.38)), roughness=0.9))
+    gq = _quad((-half, 0, half), (half, 0, half), (half, 0, -half), (-half, 0, -half), (0, 1, 0))
+    d.instances.append(identity_instance(len(d.meshes)))
+    d.meshes.append(_mesh_from_parts([(*gq, 0)], [gm]))
+    first_mesh = len(d.meshes)
+    n5 = int(round(n_meshes * level5_fraction))
+    for i in range(n_meshes):
+        mi = len(d.materials)
+        d.materials.append(material(tuple(rng.uniform(0, 1, 3)) + (1.0,), metallic=float(rng.random() < 0.3),
+                                    roughness=float(rng.uniform(0.05, 1.0))))
+        mesh = displaced_sphere_mesh(levels[1] if i < n5 else levels[0], seed * 1000 + 500 + i, 0.08, mat=0)
+        mesh.materials = [mi]
+        d.meshes.append(mesh)
+    for k in range(n_instances):
+        m = first_mesh + int(rng.integers(0, n_meshes))
+        radius = float(rng.uniform(1.5, 5.0))
+        c = rng.uniform(-half + 8, half - 8, size=2)
+        ang = rng.uniform(0, 2 * np.pi)
+        rot = np.array([[math.cos(ang), 0, math.sin(ang)], [0, 1, 0], [-math.sin(ang), 0, math.cos(ang)]])
+        model, normal = trs((c[0], radius * 0.95, c[1]), rot, radius)
+        d.instances.append((model, normal, m, -1))
+    # emissive panels: square grids of 2 n^2 triangles
+    per_panel = 512
+    n_panels = max(1, emissive_triangles // per_panel)
+    g = int(round(math.sqrt(per_panel / 2)))
+    for p in range(n_panels):
+        em = len(d.materials)
+        col = tuple(float(x) for x in rng.uniform(0.7, 1.0, 3))
+        d.materials.append(material((0.8, 0.8, 0.8, 1.0), emissive=tuple(12.0 * c for c in col) + (1.0,)))
+        P, N, U, I = _grid_panel(g, 24.0)
+        d.meshes.append(_mesh_from_parts([(P, N, U, I, 0)], [em]))
+        model, normal = trs((float(rng.uniform(-80, 80)), 35.0 + 5.0 * p, float(rng.uniform(-80, 80))), None, 1.0)
+        d.instances.append((model, normal, len(d.meshes) - 1, -1))
+    d.lights.append(dict(type=LIGHT_DIRECTIONAL, color=(1.0, 0.96, 0.9), intensity=4.0, radius=0.1,
+                         position=(0.0, 0.0, 0.0), forward=sun_forward(30.0, 35.0), inner_cone_deg=40.0,
+                         outer_cone_deg=50.0))
+    cams = []
+    for v in range(4):
+        ang = 2 * np.pi * v / 4 + rng.uniform(-0.3, 0.3)
+        dist = rng.uniform(120.0, 220.0)
+        cams.append(look_at_camera((dist * math.cos(ang), rng.uniform(15.0, 60.0), dist * math.sin(ang)),
+                                   (float(rng.uniform(-30, 30)), 0.0, float(rng.uniform(-30, 30))), fov_y=55.0,
+                                   near=1.0, far=2000.0, aperture=0.0))
+    d.camera = cams[0]
+    return d, cams

@@ -83,3 +83,21 @@ def test_feature_planes_cleared_with_the_surface(gpu_ctx):
     surf.clear()
     a0, a1 = surf.read_features()
     assert not a0.any() and not a1.any()
+
+
+def test_megakernel_writes_the_same_feature_planes(gpu_ctx):
+    from helios_amd import runtime
+    from helios_amd.abi import HH_FLAG_MEGAKERNEL
+    d = scenegen.icosphere_field(seed=3, n_spheres=6, level=3, n_leaves=120, width=96, height=64, max_ray_bounces=6)
+    scene = runtime.Scene(gpu_ctx, d)
+    planes = []
+    for flags in (0, HH_FLAG_MEGAKERNEL):
+        surf = runtime.Surface(gpu_ctx, d.width, d.height)
+        surf.enable_features()
+        it = runtime.PathIntegrator(gpu_ctx)
+        it.set_max_ray_bounces(6)
+        it.set_flags(flags)
+        it.render(scene, d.camera, surf, frames=4)
+        planes.append(surf.read_features())
+    for a, b in zip(*planes):
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
