@@ -40,7 +40,7 @@ class LightDesc(C.Structure):
 
 class SkyParams(C.Structure):
     _fields_ = [("turbidity", C.c_float), ("albedo", C.c_float), ("normalized_sun_y", C.c_float),
-                ("cube_size", C.c_uint32)]
+                ("cube_size", C.c_uint32), ("equirect_width", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -74,7 +74,8 @@ class Hit(C.Structure):
 
 class View(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("surface", C.c_void_p), ("camera", Camera), ("spp", C.c_uint32),
-                ("max_ray_bounces", C.c_uint32), ("shadow_ray_bias", C.c_float), ("first_frame", C.c_uint32)]
+                ("max_ray_bounces", C.c_uint32), ("shadow_ray_bias", C.c_float), ("first_frame", C.c_uint32),
+                ("flags", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -101,6 +102,7 @@ HH_TRACE_TERMINATE_ON_FIRST = 0x4
 HH_FLAG_NO_RADIANCE_CLAMP = 0x1
 HH_FLAG_DIRECT_ONLY = 0x2
 HH_FLAG_MEGAKERNEL = 0x4
+HH_FLAG_LIGHT_POWER = 0x8
 
 TEXTURE_RGBA8_UNORM = 0
 TEXTURE_RGBA8_SRGB = 1

@@ -634,6 +634,8 @@ hh_status hh_scene_set_sky(hh_scene s, const hh_sky_params* p) {
     HH_ARG(p->cube_size >= 1 && p->cube_size <= 4096, "sky cube size out of range");
     HH_ARG(p->turbidity >= 1.0f && p->turbidity <= 10.0f, "turbidity must be in [1, 10]");
     HH_ARG(p->albedo >= 0.0f && p->albedo <= 1.0f, "ground albedo must be in [0, 1]");
+    HH_ARG(p->equirect_width == 0 || (p->equirect_width >= 16 && p->equirect_width <= 16384 && p->equirect_width % 2 == 0),
+           "equirect_width must be 0 or even in [16, 16384]");
     s->sky = *p;
     s->dirty = true;
     return HH_OK;
@@ -918,10 +920,81 @@ hh_status hh_scene_commit(hh_scene s) {
         for (size_t i = 0; i < size_t(t.w) * t.h; i++)
             texels.push_back(make_float4(t.texels[i * 4], t.texels[i * 4 + 1], t.texels[i * 4 + 2], t.texels[i * 4 + 3]));
     }
+    // ---- power-proportional emitters (HH_FLAG_LIGHT_POWER): every emissive
+    // triangle of every instance in world space; selection probability
+    // proportional to luminance(emission) x area, accumulated in double
+    std::vector<float4> emit_tri, emit_le;
+    std::vector<float> emit_cdf;
+    {
+        std::vector<double> power;
+        for (size_t n2 = 0; n2 < s->instances.size(); n2++) {
+            const hh_instance_desc& inst = s->instances[n2];
+            const MeshHost& me = s->meshes[inst.mesh];
+            for (const hh_submesh& sm : me.sm) {
+                const uint32_t mid = inst.material_override >= 0 ? (uint32_t)inst.material_override : me.mats[sm.mat_idx];
+                const hh_material_desc& m = s->materials[mid];
+                const double le[3] = {m.emissive_value[0], m.emissive_value[1], m.emissive_value[2]};
+                const double lum = 0.2126 * le[0] + 0.7152 * le[1] + 0.0722 * le[2];
+                if (!(lum > 0.0)) continue;
+                for (uint32_t t = 0; t < sm.index_count / 3; t++) {
+                    double w[3][3], ns[3] = {0.0, 0.0, 0.0};
+                    for (int k = 0; k < 3; k++) {
+                        const hh_vertex& vx = me.v[me.idx[sm.base_index + 3 * t + k]];
+                        const float* p = vx.position;
+                        for (int r = 0; r < 3; r++) {
+                            w[k][r] = (double)inst.model[r] * p[0] + (double)inst.model[4 + r] * p[1] +
+                                      (double)inst.model[8 + r] * p[2] + (double)inst.model[12 + r];
+                            ns[r] += (double)inst.normal[r] * vx.normal[0] + (double)inst.normal[4 + r] * vx.normal[1] +
+                                     (double)inst.normal[8 + r] * vx.normal[2];
+                        }
+                    }
+                    double e1[3] = {w[1][0] - w[0][0], w[1][1] - w[0][1], w[1][2] - w[0][2]};
+                    double e2[3] = {w[2][0] - w[0][0], w[2][1] - w[0][1], w[2][2] - w[0][2]};
+                    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                           cz = e1[0] * e2[1] - e1[1] * e2[0];
+                    // the emitting side is the vertex normals' side: wind so that
+                    // (v1 - v0) x (v2 - v0) points there
+                    if (cx * ns[0] + cy * ns[1] + cz * ns[2] < 0.0) {
+                        for (int r = 0; r < 3; r++) std::swap(w[1][r], w[2][r]);
+                        cx = -cx;
+                        cy = -cy;
+                        cz = -cz;
+                    }
+                    const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+                    if (!(area > 0.0)) continue;
+                    emit_tri.push_back(make_float4((float)w[0][0], (float)w[0][1], (float)w[0][2], (float)area));
+                    emit_tri.push_back(make_float4((float)w[1][0], (float)w[1][1], (float)w[1][2], 0.0f));
+                    emit_tri.push_back(make_float4((float)w[2][0], (float)w[2][1], (float)w[2][2], 0.0f));
+                    emit_le.push_back(make_float4((float)le[0], (float)le[1], (float)le[2], 0.0f));
+                    power.push_back(lum * area);
+                }
+            }
+        }
+        double total = 0.0;
+        for (double p : power) total += p;
+        double acc = 0.0;
+        for (size_t i = 0; i < power.size(); i++) {
+            acc += power[i];
+            emit_le[i].w = (float)(power[i] / total);
+            emit_cdf.push_back(i + 1 == power.size() ? 1.0f : (float)(acc / total));
+        }
+    }
     // ---- environment: IBL, sky (directional light), or the black 2x2 default -----
     uint32_t env_size;
     float4* d_env = nullptr;
-    if (has_ibl) {
+    uint16_t* d_env_eq = nullptr;
+    const uint32_t eq_w = (!has_ibl && n_dir > 0) ? s->sky.equirect_width : 0u;
+    if (eq_w > 0) {
+        // extension: the sky as an equirectangular RGBA16F map (no cube)
+        float sd[3] = {-sun->forward[0], -sun->forward[1], -sun->forward[2]};
+        sky_coefficients(s->sky.turbidity, s->sky.albedo, s->sky.normalized_sun_y, sd, s->sky_coef);
+        HH_HIP(hipMemcpyAsync(ctx->d_coef, s->sky_coef, sizeof(s->sky_coef), hipMemcpyHostToDevice, ctx->stream));
+        HH_HIP(dalloc(s, size_t(eq_w) * (eq_w / 2) * 4 * sizeof(uint16_t), (void**)&d_env_eq));
+        HH_HIP(launch_sky_bake_equirect(ctx->stream, d_env_eq, eq_w, ctx->d_coef, sd));
+        env_size = 2;
+        HH_HIP(dalloc(s, 6 * 4 * sizeof(float4), (void**)&d_env));
+        HH_HIP(launch_fill_f4(ctx->stream, d_env, make_float4(0.f, 0.f, 0.f, 0.f), 24));
+    } else if (has_ibl) {
         env_size = s->ibl_size;
         HH_HIP(dalloc(s, size_t(6) * env_size * env_size * sizeof(float4), (void**)&d_env));
         HH_HIP(hipMemcpyAsync(d_env, s->ibl.data(), size_t(6) * env_size * env_size * sizeof(float4), hipMemcpyHostToDevice,
@@ -948,6 +1021,15 @@ hh_status hh_scene_commit(hh_scene s) {
     g.env_size = env_size;
     g.num_lights = num_lights;
     g.tlas_root = tlas_root;
+    g.env_eq = d_env_eq;
+    g.env_eq_w = eq_w;
+    g.num_area = (uint32_t)area.size();
+    g.num_emit = (uint32_t)emit_cdf.size();
+    if (!emit_cdf.empty()) {
+        HH_HIP(upload(s, emit_tri, (float4**)&g.emit_tri));
+        HH_HIP(upload(s, emit_le, (float4**)&g.emit_le));
+        HH_HIP(upload(s, emit_cdf, (float**)&g.emit_cdf));
+    }
     g.bvh_width = (uint32_t)layout;
     HH_HIP(upload(s, trav, (InstTrav**)&g.inst_trav));
     HH_HIP(upload(s, inst_data, (InstanceData**)&g.inst));
@@ -999,6 +1081,19 @@ hh_status hh_scene_read_env(hh_scene s, float* out, uint32_t* out_size) {
     *out_size = s->gpu.env_size;
     if (out) {
         HH_HIP(hipMemcpy(out, s->gpu.env, size_t(6) * s->gpu.env_size * s->gpu.env_size * sizeof(float4), hipMemcpyDeviceToHost));
+    }
+    return HH_OK;
+}
+
+hh_status hh_scene_read_env_equirect(hh_scene s, uint16_t* out, uint32_t* out_width) {
+    HH_ARG(s && out_width, "null argument");
+    hh_status st = require_committed(s);
+    if (st != HH_OK) return st;
+    *out_width = s->gpu.env_eq_w;
+    if (out && s->gpu.env_eq_w > 0) {
+        HH_HIP(hipStreamSynchronize(s->ctx->stream));
+        HH_HIP(hipMemcpy(out, s->gpu.env_eq, size_t(s->gpu.env_eq_w) * (s->gpu.env_eq_w / 2) * 4 * sizeof(uint16_t),
+                         hipMemcpyDeviceToHost));
     }
     return HH_OK;
 }
@@ -1402,7 +1497,7 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
             v.first_frame = views[i].first_frame + s;
             v.max_bounces = views[i].max_ray_bounces;
             v.shadow_bias = views[i].shadow_ray_bias;
-            v.flags = 0;
+            v.flags = views[i].flags & (HH_FLAG_NO_RADIANCE_CLAMP | HH_FLAG_DIRECT_ONLY | HH_FLAG_LIGHT_POWER);
             v.scene = scene_index[views[i].scene];
             v.slot_base = (uint32_t)total;
             v.accum = sf->d;

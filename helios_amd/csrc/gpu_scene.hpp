@@ -23,6 +23,7 @@ constexpr uint32_t HH_TRACE_TERMINATE_BIT = 0x4u;
 constexpr uint32_t HH_FLAG_NO_CLAMP_BIT = 0x1u;
 constexpr uint32_t HH_FLAG_DIRECT_ONLY_BIT = 0x2u;
 constexpr uint32_t HH_FLAG_MEGAKERNEL_BIT = 0x4u;
+constexpr uint32_t HH_FLAG_LIGHT_POWER_BIT = 0x8u;
 
 // Child reference encoding shared by all wide nodes:
 //   bit 31 clear          -> internal node index (global pool index)
@@ -128,6 +129,16 @@ struct SceneGPU {
     uint32_t num_lights;
     uint32_t tlas_root;
     uint32_t bvh_width;
+    // equirectangular environment (RGBA16F, env_eq_w x env_eq_w/2); used
+    // instead of the cube when env_eq_w > 0
+    const uint16_t* env_eq;
+    uint32_t env_eq_w;
+    // power-proportional emitters: every emissive triangle of every instance
+    uint32_t num_emit;
+    uint32_t num_area;            // area-light records at the front of `lights`
+    const float4* emit_tri;       // 3 per triangle: world v0 | area, v1, v2
+    const float4* emit_le;        // emission rgb | selection probability
+    const float* emit_cdf;        // inclusive cumulative probability
 };
 
 // Per-view launch constants (the 192-B PathTraceConsts block, reduced to the

@@ -96,8 +96,29 @@ __device__ inline void generate_ray(const ViewGPU& v, uint32_t px, uint32_t py, 
     d = normalize(focus - ap);
 }
 
+__device__ inline float half_bits(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+
+// Extension: equirectangular lookup, bilinear, repeating in x, clamped in y.
+__device__ inline f3 sample_env_equirect(const SceneGPU& S, f3 d) {
+    const int w = (int)S.env_eq_w, h = w / 2;
+    const float theta = acosf(clampf(d.y, -1.0f, 1.0f));
+    const float phi = atan2f(d.x, -d.z);
+    const float x = (phi / (2.0f * HH_PI) + 0.5f) * (float)w - 0.5f;
+    const float y = (theta / HH_PI) * (float)h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float wx = x - fx, wy = y - fy;
+    int i0 = ((int)fx % w + w) % w, i1 = (i0 + 1) % w;
+    int j0 = min(max((int)fy, 0), h - 1), j1 = min(max((int)fy + 1, 0), h - 1);
+    const uint16_t* t = S.env_eq;
+    auto px = [&](int i, int j, int c) { return half_bits(t[4 * ((size_t)j * w + i) + c]); };
+    return mk(lerpf(lerpf(px(i0, j0, 0), px(i1, j0, 0), wx), lerpf(px(i0, j1, 0), px(i1, j1, 0), wx), wy),
+              lerpf(lerpf(px(i0, j0, 1), px(i1, j0, 1), wx), lerpf(px(i0, j1, 1), px(i1, j1, 1), wx), wy),
+              lerpf(lerpf(px(i0, j0, 2), px(i1, j0, 2), wx), lerpf(px(i0, j1, 2), px(i1, j1, 2), wx), wy));
+}
+
 // Cube-map sampling: Vulkan face selection, bilinear within the face.
 __device__ inline f3 sample_env(const SceneGPU& S, f3 d) {
+    if (S.env_eq_w > 0u) return sample_env_equirect(S, d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int face;
     float sc, tc, ma;
@@ -197,18 +218,24 @@ struct ShadowRay {
     bool want;
 };
 __device__ inline ShadowRay direct_lighting(const SceneGPU& S, const Surface& p, f3 ray_dir, f3 T, uint32_t depth,
-                                            float shadow_bias, RNG& rng) {
+                                            float shadow_bias, uint32_t vflags, RNG& rng) {
     ShadowRay sr;
     sr.want = false;
     sr.flags = depth == 0 ? 0u : (HH_TRACE_OPAQUE_BIT | HH_TRACE_TERMINATE_BIT);
     sr.tmin = 0.0001f;
     sr.tmax = 10000.0f;
     sr.o = p.pos + p.normal * shadow_bias;
-    uint32_t nl = S.num_lights;
+    // Reference: uniform pick among the packed lights. Power mode (extension):
+    // uniform among [all emitters as one entry] + the lights after the
+    // area-light records; type 5 = an emitter chosen by power.
+    const bool power = (vflags & HH_FLAG_LIGHT_POWER_BIT) != 0u;
+    const uint32_t nemit = power && S.num_emit > 0u ? 1u : 0u;
+    const uint32_t nl = power ? S.num_lights - S.num_area + nemit : S.num_lights;
     uint32_t li = next_uint(rng, nl);
     if (nl == 0) return sr;
-    const LightData& L = S.lights[li];
-    uint32_t type = (uint32_t)L.d0[0];
+    const bool emitter = nemit != 0u && li == 0u;
+    const LightData& L = S.lights[emitter ? 0u : (power ? li - nemit + S.num_area : li)];
+    uint32_t type = emitter ? 5u : (uint32_t)L.d0[0];
     f3 Wi = mk(0.f, 0.f, 0.f), Li = mk(0.f, 0.f, 0.f);
     float pdf = 0.0f;
     if (type == 0u) {
@@ -278,6 +305,37 @@ __device__ inline ShadowRay direct_lighting(const SceneGPU& S, const Surface& p,
         Li = mk(lm.emissive[0], lm.emissive[1], lm.emissive[2]);
         Wi = -ldir;
         pdf = dsq / fmaxf(HH_EPS, ct * area);
+    } else if (type == 5u) {
+        // emitter triangle by power (inverse CDF), uniform point on it;
+        // pdf in solid angle = P(tri) / area * d^2 / cos (front face only)
+        const float u = next_float(rng);
+        uint32_t lo = 0, hi = S.num_emit - 1u;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (u < S.emit_cdf[mid]) hi = mid;
+            else lo = mid + 1u;
+        }
+        const float4 a = S.emit_tri[3 * (size_t)lo], b = S.emit_tri[3 * (size_t)lo + 1], c3 = S.emit_tri[3 * (size_t)lo + 2];
+        const float4 le = S.emit_le[lo];
+        float b0 = next_float(rng);
+        float b1 = next_float(rng);
+        float su0 = sqrtf(b0);
+        float bx = 1.0f - su0, by = b1 * su0;
+        f3 P0 = ld3(a), P1 = ld3(b), P2 = ld3(c3);
+        float w0 = 1.0f - bx - by;
+        f3 lp = P0 * w0 + P1 * bx + P2 * by;
+        f3 ln = normalize(cross(P1 - P0, P2 - P0));
+        f3 ldir = p.pos - lp;
+        float dsq = dot(ldir, ldir);
+        if (dsq == 0.0f) return sr;
+        float dist = sqrtf(dsq);
+        ldir = ldir / dist;
+        sr.tmax = fmaxf(0.0f, dist - HH_EPS);
+        float ct = dot(ln, ldir);
+        if (!(ct > 0.0f)) return sr;
+        Li = mk(le.x, le.y, le.z);
+        Wi = -ldir;
+        pdf = le.w / a.w * dsq / ct;
     }
     f3 Wo = -ray_dir;
     f3 Wh = normalize(Wo + Wi);
@@ -466,7 +524,7 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
                 populate(S, __float_as_uint(hh.w), hit_tri[q], hh.y, hh.z, p);
                 if (features) write_features(feat, feat_stride, slot, p.albedo, hh.x, p.normal, false);
                 if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
-                sr = direct_lighting(S, p, dir, Tp, depth, V.shadow_bias, rng);
+                sr = direct_lighting(S, p, dir, Tp, depth, V.shadow_bias, V.flags, rng);
                 if ((V.flags & HH_FLAG_DIRECT_ONLY_BIT) == 0u && depth + 1 < V.max_bounces) {
                     f3 Wo = -dir, Wi;
                     float pdf;
@@ -648,7 +706,7 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
             populate(S, h.inst, h.tri, h.u, h.v, p);
             if (features) write_features(feat, feat_stride, slot, p.albedo, h.t, p.normal, false);
             if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
-            ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, rng);
+            ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, V.flags, rng);
             if (sr.want) {
                 HitRec sh;
                 n_sh++;
@@ -755,6 +813,41 @@ __global__ void k_tonemap(const float4* __restrict__ src, uint32_t n, float expo
     dst[i] = make_uchar4(o[0], o[1], o[2], 255);
 }
 
+// procedural_sky.frag:48-75 for one unit direction.
+__device__ inline float4 hosek_eval(f3 d, const float* __restrict__ coef, float sx, float sy, float sz) {
+    float cos_theta = clampf(d.y, 0.0f, 1.0f);
+    float cos_gamma = clampf(d.x * sx + d.y * sy + d.z * sz, 0.0f, 1.0f);
+    float gamma = acosf(cos_gamma);
+    float out[3];
+    for (int c = 0; c < 3; c++) {
+        float A = coef[0 + c], B = coef[3 + c], C = coef[6 + c], D = coef[9 + c], E = coef[12 + c], F = coef[15 + c],
+              G = coef[18 + c], H = coef[21 + c], I = coef[24 + c], Z = coef[27 + c];
+        float chi = (1.0f + cos_gamma * cos_gamma) / powf(1.0f + H * H - 2.0f * cos_gamma * H, 1.5f);
+        float hw = (1.0f + A * expf(B / (cos_theta + 0.01f))) *
+                   (C + D * expf(E * gamma) + F * (cos_gamma * cos_gamma) + G * chi + I * sqrtf(cos_theta));
+        out[c] = Z * hw;
+    }
+    return make_float4(out[0], out[1], out[2], 1.0f);
+}
+
+// Extension: the sky baked into an equirectangular RGBA16F map (texel
+// (i, j): theta = pi (j + 0.5) / H, phi = 2 pi ((i + 0.5) / W - 0.5)).
+__global__ void k_sky_bake_equirect(uint16_t* __restrict__ texels, uint32_t w, const float* __restrict__ coef, float sx,
+                                    float sy, float sz) {
+    const uint32_t h = w / 2;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    const float theta = HH_PI * (((float)y + 0.5f) / (float)h);
+    const float phi = 2.0f * HH_PI * (((float)x + 0.5f) / (float)w - 0.5f);
+    const f3 d = mk(sinf(theta) * sinf(phi), cosf(theta), -sinf(theta) * cosf(phi));
+    const float4 c = hosek_eval(d, coef, sx, sy, sz);
+    texels[4 * (size_t)i + 0] = __builtin_bit_cast(uint16_t, (_Float16)c.x);
+    texels[4 * (size_t)i + 1] = __builtin_bit_cast(uint16_t, (_Float16)c.y);
+    texels[4 * (size_t)i + 2] = __builtin_bit_cast(uint16_t, (_Float16)c.z);
+    texels[4 * (size_t)i + 3] = __builtin_bit_cast(uint16_t, (_Float16)1.0f);
+}
+
 // procedural_sky.frag:48-75 evaluated at every cube texel centre.
 __global__ void k_sky_bake(float4* __restrict__ faces, uint32_t n, const float* __restrict__ coef, float sx, float sy, float sz) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -773,19 +866,7 @@ __global__ void k_sky_bake(float4* __restrict__ faces, uint32_t n, const float* 
         default: d = mk(-sc, -tc, -1.0f); break;
     }
     d = normalize(d);
-    float cos_theta = clampf(d.y, 0.0f, 1.0f);
-    float cos_gamma = clampf(d.x * sx + d.y * sy + d.z * sz, 0.0f, 1.0f);
-    float gamma = acosf(cos_gamma);
-    float out[3];
-    for (int c = 0; c < 3; c++) {
-        float A = coef[0 + c], B = coef[3 + c], C = coef[6 + c], D = coef[9 + c], E = coef[12 + c], F = coef[15 + c],
-              G = coef[18 + c], H = coef[21 + c], I = coef[24 + c], Z = coef[27 + c];
-        float chi = (1.0f + cos_gamma * cos_gamma) / powf(1.0f + H * H - 2.0f * cos_gamma * H, 1.5f);
-        float hw = (1.0f + A * expf(B / (cos_theta + 0.01f))) *
-                   (C + D * expf(E * gamma) + F * (cos_gamma * cos_gamma) + G * chi + I * sqrtf(cos_theta));
-        out[c] = Z * hw;
-    }
-    faces[i] = make_float4(out[0], out[1], out[2], 1.0f);
+    faces[i] = hosek_eval(d, coef, sx, sy, sz);
 }
 
 __global__ void k_fill_f4(float4* p, float4 v, uint32_t n) {
@@ -944,6 +1025,12 @@ hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float e
 hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const float* d_coef, const float sun[3]) {
     uint32_t total = 6u * size * size;
     k_sky_bake<<<cdiv(total, 256), 256, 0, st>>>(faces, size, d_coef, sun[0], sun[1], sun[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_bake_equirect(hipStream_t st, uint16_t* texels, uint32_t width, const float* d_coef, const float sun[3]) {
+    const uint32_t total = width * (width / 2);
+    k_sky_bake_equirect<<<cdiv(total, 256), 256, 0, st>>>(texels, width, d_coef, sun[0], sun[1], sun[2]);
     return hipGetLastError();
 }
 

@@ -127,6 +127,7 @@ hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, co
                                uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
 hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst);
 hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const float* d_coef, const float sun[3]);
+hipError_t launch_sky_bake_equirect(hipStream_t st, uint16_t* texels, uint32_t width, const float* d_coef, const float sun[3]);
 
 // Host Hosek-Wilkie fit (sky_host.cpp)
 void sky_coefficients(float turbidity, float albedo, float normalized_sun_y, const float sun_dir[3], float out[30]);

@@ -55,6 +55,7 @@ _SIGS = {
     "hh_scene_read_lights": [C.c_void_p, C.c_void_p, C.c_uint32],
     "hh_scene_read_env": [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)],
     "hh_scene_sky_coefficients": [C.c_void_p, C.c_void_p],
+    "hh_scene_read_env_equirect": [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)],
     "hh_scene_bvh_stats": [C.c_void_p, C.c_void_p],
     "hh_surface_create": [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)],
     "hh_surface_destroy": [C.c_void_p],
@@ -245,6 +246,16 @@ class Scene:
         check(lib().hh_scene_read_env(self.h, ptr(out), C.byref(n)))
         return out
 
+    def env_equirect(self):
+        """Equirectangular sky as float16 (H, W, 4), or None for the cube."""
+        w = C.c_uint32()
+        check(lib().hh_scene_read_env_equirect(self.h, None, C.byref(w)))
+        if not w.value:
+            return None
+        out = np.zeros((w.value // 2, w.value, 4), np.uint16)
+        check(lib().hh_scene_read_env_equirect(self.h, ptr(out), C.byref(w)))
+        return out.view(np.float16)
+
     def sky_coefficients(self):
         out = np.zeros(30, np.float32)
         check(lib().hh_scene_sky_coefficients(self.h, ptr(out)))
@@ -402,16 +413,8 @@ class PathIntegrator:
 
 
 def render_batch(ctx, views):
-    """views: list of dict(scene, surface, camera, spp, max_ray_bounces, shadow_ray_bias, first_frame)."""
-    arr = (abi.View * len(views))()
-    for i, v in enumerate(views):
-        arr[i].scene = v["scene"].h.value
-        arr[i].surface = v["surface"].h.value
-        arr[i].camera = camera_struct(v["camera"])
-        arr[i].spp = v.get("spp", 1)
-        arr[i].max_ray_bounces = v.get("max_ray_bounces", 8)
-        arr[i].shadow_ray_bias = v.get("shadow_ray_bias", 0.0)
-        arr[i].first_frame = v.get("first_frame", 0)
+    """views: list of dict(scene, surface, camera, spp, max_ray_bounces, shadow_ray_bias, first_frame, flags)."""
+    arr = _view_array(views)
     check(lib().hh_render_batch(ctx.h, arr, len(views)))
 
 
@@ -425,6 +428,7 @@ def _view_array(views):
         arr[i].max_ray_bounces = v.get("max_ray_bounces", 8)
         arr[i].shadow_ray_bias = v.get("shadow_ray_bias", 0.0)
         arr[i].first_frame = v.get("first_frame", 0)
+        arr[i].flags = v.get("flags", 0)
     return arr
 
 

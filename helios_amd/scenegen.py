@@ -476,15 +476,17 @@ def _grid_panel(n, size):
 
 
 def combined_workload(seed=0, width=3840, height=2160, n_meshes=40, n_instances=1000, max_ray_bounces=16, spp=2,
-                      levels=(4, 5), level5_fraction=0.35, emissive_triangles=1024):
+                      levels=(4, 5), level5_fraction=0.35, emissive_triangles=1024, equirect_width=0):
     """The combined 4K workload: n_instances instances of n_meshes seeded
     displaced meshes (levels 4 and 5: ~10 M instanced triangles) in one TLAS
     on a 400 x 400 ground, emissive panels with emissive_triangles triangles in
     total (area lights), the Hosek-Wilkie sky with its sun and 16 bounces.
-    Returns (SceneDesc, [4 cameras])."""
+    equirect_width > 0 bakes the sky into an equirectangular RGBA16F map of
+    that width (the workload's 2048 x 1024 HDR environment) instead of the
+    reference's cube. Returns (SceneDesc, [4 cameras])."""
     rng = np.random.Generator(np.random.PCG64(seed + 104729))
     d = SceneDesc(width=width, height=height, spp=spp, max_ray_bounces=max_ray_bounces)
-    d.sky = (3.0, 0.1, 1.15, 512)
+    d.sky = (3.0, 0.1, 1.15, 512, int(equirect_width))
     half = 200.0
     gm = len(d.materials)
     d.materials.append(material(linear_albedo((0.4, 0.4, 0.38)), roughness=0.9))

@@ -146,6 +146,13 @@ typedef struct hh_sky_params {
     float albedo;
     float normalized_sun_y;
     uint32_t cube_size;     /* 512 in the reference (hosek_wilkie_sky_model.cpp:17) */
+    /* Extension (not in the reference): > 0 bakes the sky into an
+     * equirectangular RGBA16F map of equirect_width x equirect_width/2 texels
+     * instead of the cube. Texel (i, j) holds direction
+     * (sin t sin p, cos t, -sin t cos p), t = pi (j + 0.5) / H,
+     * p = 2 pi ((i + 0.5) / W - 0.5); lookups are bilinear, repeating in i and
+     * clamped in j. 0 = the reference cube. */
+    uint32_t equirect_width;
 } hh_sky_params;
 
 /* CameraNode (include/resource/scene.h:247-282). orientation is a quaternion
@@ -175,6 +182,12 @@ typedef struct hh_settings {
 #define HH_FLAG_NO_RADIANCE_CLAMP 0x1u   /* diagnostic: skip the min(L, 1) clamp */
 #define HH_FLAG_DIRECT_ONLY 0x2u         /* DIRECT_LIGHTING_INTEGRATOR variant */
 #define HH_FLAG_MEGAKERNEL 0x4u          /* one kernel per frame instead of the wavefront */
+/* Extension (not in the reference, whose area lights are Q2/Q8-quirked):
+ * next-event estimation picks uniformly among [all emissive triangles of all
+ * instances as one entry, env, directional, point, spot lights], and inside
+ * the emissive entry a triangle with probability proportional to its power
+ * (luminance(emission) x world area), then a uniform point on it. */
+#define HH_FLAG_LIGHT_POWER 0x8u
 
 /* The 192-B PathTraceConsts push-constant block (path_integrator.cpp:11-28). */
 typedef struct hh_launch_consts {
@@ -254,6 +267,9 @@ hh_status hh_scene_num_lights(hh_scene scene, uint32_t* out);
 hh_status hh_scene_read_lights(hh_scene scene, float* out_records, uint32_t capacity);
 /* Copies the baked environment cube (6 x size x size RGBA32F) to host. */
 hh_status hh_scene_read_env(hh_scene scene, float* out_faces, uint32_t* out_size);
+/* The equirectangular sky (RGBA16F bits, W x W/2 x 4) when equirect_width > 0;
+ * *out_width = 0 when the scene uses the cube. out may be null to query. */
+hh_status hh_scene_read_env_equirect(hh_scene scene, uint16_t* out, uint32_t* out_width);
 /* Sky coefficients A..I, Z (10 x vec3, as float) used by the bake. */
 hh_status hh_scene_sky_coefficients(hh_scene scene, float out[30]);
 /* Host-only: the Hosek-Wilkie coefficient fit (A..I, Z as 10 x rgb) for a sun
@@ -321,6 +337,7 @@ typedef struct hh_view {
     uint32_t max_ray_bounces;
     float shadow_ray_bias;
     uint32_t first_frame;      /* num_frames of the first sample */
+    uint32_t flags;            /* HH_FLAG_NO_RADIANCE_CLAMP / DIRECT_ONLY / LIGHT_POWER */
 } hh_view;
 /* Renders spp progressive samples of every view, up to frames_in_flight
  * samples per view per wavefront pass (and at most queue_capacity path slots

@@ -54,6 +54,8 @@ def lib():
         L.orc_scene_read_materials.restype = C.c_uint32
         L.orc_scene_env_size.argtypes = [vp]
         L.orc_scene_env_size.restype = C.c_uint32
+        L.orc_scene_env_equirect.argtypes = [vp, vp]
+        L.orc_scene_env_equirect.restype = C.c_uint32
         L.orc_scene_read_env.argtypes = [vp, vp]
         L.orc_scene_sky_coefficients.argtypes = [vp, vp]
         L.orc_sky_coefficients.argtypes = [C.POINTER(abi.SkyParams), vp, vp]
@@ -161,6 +163,15 @@ class OracleScene:
         out = np.zeros((6, n, n, 4), np.float32)
         lib().orc_scene_read_env(self.h, ptr(out))
         return out
+
+    def env_equirect(self):
+        """Equirectangular sky as float16 (H, W, 4), or None for the cube."""
+        w = lib().orc_scene_env_equirect(self.h, None)
+        if not w:
+            return None
+        out = np.zeros((w // 2, w, 4), np.uint16)
+        lib().orc_scene_env_equirect(self.h, ptr(out))
+        return out.view(np.float16)
 
     def sky_coefficients(self):
         out = np.zeros(30, np.float32)

@@ -250,6 +250,65 @@ static V3<R> sample_cube(const std::vector<float>& faces, uint32_t size, V3<R> d
     return V3<R>(out[0], out[1], out[2]);
 }
 
+// IEEE binary16 conversions (round to nearest even; finite inputs).
+static uint16_t float_to_half(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const int32_t exp = (int32_t)((x >> 23) & 0xFF) - 127 + 15;
+    uint32_t mant = x & 0x7FFFFFu;
+    if (exp >= 31) return (uint16_t)(sign | 0x7C00u);               // overflow -> inf
+    if (exp <= 0) {                                                  // subnormal or zero
+        if (exp < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        const int shift = 14 - exp;
+        uint32_t h = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)exp << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;          // carries into the exponent correctly
+    return (uint16_t)(sign | h);
+}
+static float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t exp = (h >> 10) & 0x1Fu, mant = h & 0x3FFu;
+    uint32_t x;
+    if (exp == 0) {
+        if (mant == 0) x = sign;
+        else {
+            int e = -1;
+            do { mant <<= 1; e++; } while (!(mant & 0x400u));
+            x = sign | ((uint32_t)(127 - 15 - e) << 23) | ((mant & 0x3FFu) << 13);
+        }
+    } else if (exp == 31) x = sign | 0x7F800000u | (mant << 13);
+    else x = sign | ((exp - 15 + 127) << 23) | (mant << 13);
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+// Extension: equirectangular lookup (bilinear, repeat in x, clamp in y).
+template <typename R>
+static V3<R> sample_equirect(const std::vector<uint16_t>& tex, uint32_t w, V3<R> d) {
+    const int64_t W = w, H = w / 2;
+    R theta = std::acos(gclamp(d.y, R(-1), R(1)));
+    R phi = std::atan2(d.x, -d.z);
+    R x = (phi / (R(2) * kPi<R>()) + R(0.5)) * R(W) - R(0.5);
+    R y = (theta / kPi<R>()) * R(H) - R(0.5);
+    R fx = std::floor(x), fy = std::floor(y);
+    R wx = x - fx, wy = y - fy;
+    int64_t i0 = (((int64_t)fx % W) + W) % W, i1 = (i0 + 1) % W;
+    auto cl = [H](int64_t j) { return j < 0 ? 0 : (j >= H ? H - 1 : j); };
+    int64_t j0 = cl((int64_t)fy), j1 = cl((int64_t)fy + 1);
+    auto px = [&](int64_t i, int64_t j, int c) { return R(half_to_float(tex[((size_t)j * W + i) * 4 + c])); };
+    R out[3];
+    for (int c = 0; c < 3; c++) out[c] = lerp_(lerp_(px(i0, j0, c), px(i1, j0, c), wx), lerp_(px(i0, j1, c), px(i1, j1, c), wx), wy);
+    return V3<R>(out[0], out[1], out[2]);
+}
+
 // ===========================================================================
 // Hosek-Wilkie sky (hosek_wilkie_sky_model.cpp:41-75, 658-701;
 // procedural_sky.frag:48-75). Dataset: helios_amd/data/hosek_rgb.bin.
@@ -490,11 +549,90 @@ void commit(Scene& s) {
         }
     // num_lights (scene.cpp:891-899)
     s.num_lights = (uint32_t)area.size() + n_dir + n_spot + n_point + ((has_ibl || n_dir > 0) ? 1u : 0u);
+    s.num_area = (uint32_t)area.size();
+    // Extension (HH_FLAG_LIGHT_POWER): every emissive triangle of every
+    // instance, world vertices in double, P(tri) = luminance(emission) x area
+    // / total, inclusive CDF (last entry exactly 1).
+    s.emit_v.clear(); s.emit_area.clear(); s.emit_p.clear(); s.emit_cdf.clear(); s.emit_le.clear();
+    {
+        std::vector<double> power;
+        for (const hh_instance_desc& inst : s.instances) {
+            const Mesh& me = s.meshes[inst.mesh];
+            for (const hh_submesh& sm : me.submeshes) {
+                const uint32_t mid = inst.material_override >= 0 ? (uint32_t)inst.material_override : me.materials[sm.mat_idx];
+                const hh_material_desc& m = s.materials[mid];
+                const double le[3] = {m.emissive_value[0], m.emissive_value[1], m.emissive_value[2]};
+                const double lum = 0.2126 * le[0] + 0.7152 * le[1] + 0.0722 * le[2];
+                if (!(lum > 0.0)) continue;
+                for (uint32_t t = 0; t < sm.index_count / 3; t++) {
+                    double w[3][3], ns[3] = {0.0, 0.0, 0.0};
+                    for (int k = 0; k < 3; k++) {
+                        const hh_vertex& vx = me.vertices[me.indices[sm.base_index + 3 * t + k]];
+                        const float* p = vx.position;
+                        for (int r = 0; r < 3; r++) {
+                            w[k][r] = (double)inst.model[r] * p[0] + (double)inst.model[4 + r] * p[1] +
+                                      (double)inst.model[8 + r] * p[2] + (double)inst.model[12 + r];
+                            ns[r] += (double)inst.normal[r] * vx.normal[0] + (double)inst.normal[4 + r] * vx.normal[1] +
+                                     (double)inst.normal[8 + r] * vx.normal[2];
+                        }
+                    }
+                    double e1[3] = {w[1][0] - w[0][0], w[1][1] - w[0][1], w[1][2] - w[0][2]};
+                    double e2[3] = {w[2][0] - w[0][0], w[2][1] - w[0][1], w[2][2] - w[0][2]};
+                    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                           cz = e1[0] * e2[1] - e1[1] * e2[0];
+                    // the emitting side is the vertex normals' side: wind so that
+                    // (v1 - v0) x (v2 - v0) points there
+                    if (cx * ns[0] + cy * ns[1] + cz * ns[2] < 0.0) {
+                        for (int r = 0; r < 3; r++) std::swap(w[1][r], w[2][r]);
+                        cx = -cx;
+                        cy = -cy;
+                        cz = -cz;
+                    }
+                    const double a = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+                    if (!(a > 0.0)) continue;
+                    for (int k = 0; k < 3; k++)
+                        for (int r = 0; r < 3; r++) s.emit_v.push_back((float)w[k][r]);
+                    s.emit_area.push_back((float)a);
+                    for (int c = 0; c < 3; c++) s.emit_le.push_back((float)le[c]);
+                    power.push_back(lum * a);
+                }
+            }
+        }
+        double total = 0.0, acc = 0.0;
+        for (double p : power) total += p;
+        for (size_t i = 0; i < power.size(); i++) {
+            acc += power[i];
+            s.emit_p.push_back((float)(power[i] / total));
+            s.emit_cdf.push_back(i + 1 == power.size() ? 1.0f : (float)(acc / total));
+        }
+    }
     // Environment: IBL, else the sky when a directional light exists, else the
     // 2x2 black default cube (scene.cpp:1149-1161; vk.cpp:3589-3610).
     if (has_ibl) {
         s.env = s.ibl;
         s.env_size = s.ibl_size;
+    } else if (n_dir > 0 && s.sky.equirect_width > 0) {
+        // extension: equirectangular RGBA16F sky, texel centres as the GPU bake
+        const hh_light_desc* sun = nullptr;
+        for (auto& l : s.lights)
+            if (l.type == HH_LIGHT_DIRECTIONAL) { sun = &l; break; }
+        float sd[3] = {-sun->forward[0], -sun->forward[1], -sun->forward[2]};
+        sky_coefficients(s.sky, sd, s.sky_coef);
+        const uint32_t w = s.sky.equirect_width, h = w / 2;
+        s.env_eq_w = w;
+        s.env_eq.assign(size_t(w) * h * 4, 0);
+        V3<float> sunv(sd[0], sd[1], sd[2]);
+        for (uint32_t y = 0; y < h; y++)
+            for (uint32_t x = 0; x < w; x++) {
+                const float theta = kPi<float>() * (((float)y + 0.5f) / (float)h);
+                const float phi = 2.0f * kPi<float>() * (((float)x + 0.5f) / (float)w - 0.5f);
+                V3<float> d(std::sin(theta) * std::sin(phi), std::cos(theta), -std::sin(theta) * std::cos(phi));
+                V3<float> c = hosek_sky_rgb<float>(d, sunv, s.sky_coef);
+                uint16_t* px = &s.env_eq[(size_t(y) * w + x) * 4];
+                px[0] = float_to_half(c.x); px[1] = float_to_half(c.y); px[2] = float_to_half(c.z); px[3] = float_to_half(1.0f);
+            }
+        s.env_size = 2;
+        s.env.assign(6 * 2 * 2 * 4, 0.f);
     } else if (n_dir > 0) {
         const hh_light_desc* sun = nullptr;
         for (auto& l : s.lights)
@@ -982,7 +1120,10 @@ struct Tracer {
     Stats st;
     uint32_t rr_start = 0;  // settings.rr_start_depth (0 = the reference: RR at every bounce)
 
-    V3<R> env(V3<R> d) const { return sample_cube<R>(s.env, s.env_size, d); }
+    V3<R> env(V3<R> d) const {
+        if (s.env_eq_w > 0) return sample_equirect<R>(s.env_eq, s.env_eq_w, d);
+        return sample_cube<R>(s.env, s.env_size, d);
+    }
 
     // sample_light (path_trace_rchit.glsl:284-451); returns Li * visibility.
     V3<R> sample_light(const Surface<R>& p, const LightData& light, uint32_t depth, RNG& rng, V3<R>& Wi, R& pdf) {
@@ -1063,13 +1204,54 @@ struct Tracer {
         return h.found ? Li * R(0) : Li;
     }
 
+    // Extension (HH_FLAG_LIGHT_POWER): an emissive triangle by power
+    // (first CDF entry above u), a uniform point on it, front faces only;
+    // pdf in solid angle = P(tri) / area * d^2 / cos.
+    V3<R> sample_emitter(const Surface<R>& p, uint32_t depth, RNG& rng, V3<R>& Wi, R& pdf) {
+        uint32_t ray_flags = (depth == 0) ? 0u : (HH_TRACE_OPAQUE | HH_TRACE_TERMINATE_ON_FIRST);
+        R tmin = R(0.0001), tmax = R(10000.0);
+        V3<R> origin = p.vertex.pos + p.normal * shadow_bias;
+        const float u = next_float(rng);
+        size_t t = (size_t)(std::upper_bound(s.emit_cdf.begin(), s.emit_cdf.end(), u) - s.emit_cdf.begin());
+        if (t >= s.emit_cdf.size()) t = s.emit_cdf.size() - 1;
+        R b0 = next_float(rng), b1 = next_float(rng);
+        R su0 = std::sqrt(b0);
+        R bx = R(1) - su0, by = b1 * su0;
+        const float* v = &s.emit_v[9 * t];
+        V3<R> P0(v[0], v[1], v[2]), P1(v[3], v[4], v[5]), P2(v[6], v[7], v[8]);
+        R w0 = R(1) - bx - by;
+        V3<R> lp = P0 * w0 + P1 * bx + P2 * by;
+        V3<R> ln = normalize(cross(P1 - P0, P2 - P0));
+        V3<R> ldir = p.vertex.pos - lp;
+        R dsq = dot(ldir, ldir);
+        if (dsq == R(0)) { pdf = 0; return V3<R>(0, 0, 0); }
+        R dist = std::sqrt(dsq);
+        ldir = ldir / dist;
+        tmax = gmax(R(0), dist - kEps<R>());
+        R ct = dot(ln, ldir);
+        if (!(ct > R(0))) { pdf = 0; return V3<R>(0, 0, 0); }
+        V3<R> Li(R(s.emit_le[3 * t]), R(s.emit_le[3 * t + 1]), R(s.emit_le[3 * t + 2]));
+        Wi = -ldir;
+        pdf = R(s.emit_p[t]) / R(s.emit_area[t]) * dsq / ct;
+        st.shadow++;
+        Hit<R> h = trace<R>(s, origin, Wi, tmin, tmax, ray_flags, false, brute);
+        return h.found ? Li * R(0) : Li;
+    }
+
     V3<R> direct_lighting(const Surface<R>& p, V3<R> ray_dir, V3<R> T, uint32_t depth, RNG& rng) {  // :455-483
         V3<R> L(0, 0, 0);
-        uint32_t li = next_uint(rng, num_lights);
-        const LightData& light = s.light_data[li];
+        // reference: uniform among the packed lights; power mode: uniform among
+        // [all emitters as one entry] + the lights after the area records
+        const bool power = (flags & HH_FLAG_LIGHT_POWER) != 0u;
+        const uint32_t nemit = power && !s.emit_cdf.empty() ? 1u : 0u;
+        const uint32_t nl = power ? num_lights - s.num_area + nemit : num_lights;
+        uint32_t li = next_uint(rng, nl);
+        if (nl == 0) return L;
         V3<R> Wo = -ray_dir, Wi(0, 0, 0);
         R pdf = 0;
-        V3<R> Li = sample_light(p, light, depth, rng, Wi, pdf);
+        V3<R> Li;
+        if (nemit && li == 0) Li = sample_emitter(p, depth, rng, Wi, pdf);
+        else Li = sample_light(p, s.light_data[power ? li - nemit + s.num_area : li], depth, rng, Wi, pdf);
         V3<R> Wh = normalize(Wo + Wi);
         V3<R> brdf = evaluate_uber(p, Wo, Wh, Wi);
         R ct = gclamp(dot(p.normal, Wi), R(0), R(1));
@@ -1077,7 +1259,7 @@ struct Tracer {
             if (pdf == R(0)) L = T * brdf * ct * Li;
             else L = (T * brdf * ct * Li) / pdf;
         }
-        return L * R(num_lights);
+        return L * R(nl);
     }
 
     // payload: L, T, depth, rng. Returns L of this ray.
@@ -1299,6 +1481,11 @@ uint32_t orc_scene_read_materials(void* sp, void* out, uint32_t cap) {
     return n;
 }
 uint32_t orc_scene_env_size(void* sp) { return ((Scene*)sp)->env_size; }
+uint32_t orc_scene_env_equirect(void* sp, uint16_t* out) {
+    const Scene& s = *(Scene*)sp;
+    if (out && s.env_eq_w) std::memcpy(out, s.env_eq.data(), s.env_eq.size() * 2);
+    return s.env_eq_w;
+}
 void orc_scene_read_env(void* sp, float* out) {
     Scene& s = *(Scene*)sp;
     std::memcpy(out, s.env.data(), s.env.size() * 4);

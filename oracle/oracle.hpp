@@ -158,7 +158,7 @@ struct Scene {
     std::vector<hh_light_desc> lights;
     std::vector<float> ibl;  // 6 * size * size * 4
     uint32_t ibl_size = 0;
-    hh_sky_params sky{4.0f, 0.1f, 1.15f, 512};
+    hh_sky_params sky{4.0f, 0.1f, 1.15f, 512, 0};
     // packed
     std::vector<MaterialData> mat_data;
     std::vector<int32_t> tex_slots;          // descriptor slot -> texture id
@@ -170,6 +170,15 @@ struct Scene {
     // environment cube (float texels, 6 faces), size env_size
     std::vector<float> env;
     uint32_t env_size = 0;
+    // extension: equirectangular RGBA16F sky (bits), width env_eq_w (0 = cube)
+    std::vector<uint16_t> env_eq;
+    uint32_t env_eq_w = 0;
+    // extension: power-proportional emitters (every emissive triangle of
+    // every instance): world vertices, area, emission, probability, CDF
+    std::vector<float> emit_v;     // 9 per triangle
+    std::vector<float> emit_area, emit_p, emit_cdf;
+    std::vector<float> emit_le;    // 3 per triangle
+    uint32_t num_area = 0;         // area-light records at the front of light_data
     float sky_coef[30] = {0};
     bool committed = false;
 };

@@ -289,3 +289,66 @@ def test_rr_start_depth_matches_oracle(gpu_ctx, cornell, cornell_pair):
     # and it is a different estimator from RR at every bounce
     base = o.render(frames=16)
     assert relmse(base, ref) > 1e-5
+
+
+@pytest.fixture(scope="module")
+def combined_pair(gpu_ctx, oracle_mod):
+    """The combined workload's scene (1,000 instances of 40 meshes, emissive
+    panels, sky + sun, 16 bounces) at a reduced image size."""
+    from helios_amd import runtime
+    d, cams = scenegen.combined_workload(width=480, height=270)
+    return d, cams, runtime.Scene(gpu_ctx, d), oracle_mod.OracleScene(d)
+
+
+def test_combined_workload_primary_ids_bit_exact(combined_pair):
+    d, cams, g, o = combined_pair
+    for cam in cams[:2]:
+        hg = g.primary_hits(cam, d.width, d.height, 0)
+        ho = o.primary_hits(camera=cam, frame=0, brute=False)
+        bad = id_mismatch(hg, ho)
+        assert bad.mean() <= EDGE_TIE_FRACTION, f"{bad.sum()} mismatching pixels"
+        both = ~bad & (ho["instance"] != 0xFFFFFFFF)
+        assert np.array_equal(hg["t"][both], ho["t"][both])
+
+
+def test_combined_workload_radiance_matches_oracle(gpu_ctx, combined_pair):
+    """Instanced geometry, area lights from the emissive panels, 16 bounces:
+    GPU vs the FP32 transcription on a crop, same RNG streams."""
+    d, cams, g, o = combined_pair
+    img_g, _ = _render_gpu(gpu_ctx, g, d, frames=16, camera=cams[1])
+    crop = (200, 100, 248, 148)
+    x0, y0, x1, y1 = crop
+    img_o = o.render(camera=cams[1], frames=16, crop=crop)
+    assert relmse(img_g[y0:y1, x0:x1], img_o[y0:y1, x0:x1]) < 1e-4
+
+
+@pytest.fixture(scope="module")
+def combined_ext_pair(gpu_ctx, oracle_mod):
+    """The combined workload with its two extensions: an equirectangular
+    RGBA16F sky and power-proportional emitter sampling."""
+    from helios_amd import runtime
+    d, cams = scenegen.combined_workload(width=480, height=270, equirect_width=2048)
+    return d, cams, runtime.Scene(gpu_ctx, d), oracle_mod.OracleScene(d)
+
+
+def test_equirect_sky_bake_matches_oracle(combined_ext_pair):
+    _, _, g, o = combined_ext_pair
+    eg, eo = g.env_equirect(), o.env_equirect()
+    assert eg is not None and eg.shape == (1024, 2048, 4)
+    # both round the same float evaluation to half; the float sky terms may
+    # differ in the last bits (transcendentals), so allow one half ulp
+    ulp = np.abs(eg.view(np.int16).astype(np.int32) - eo.view(np.int16).astype(np.int32))
+    assert ulp.max() <= 1 and (ulp == 0).mean() > 0.99
+
+
+def test_light_power_and_equirect_match_oracle(gpu_ctx, combined_ext_pair):
+    from helios_amd.abi import HH_FLAG_LIGHT_POWER
+    d, cams, g, o = combined_ext_pair
+    img_g, _ = _render_gpu(gpu_ctx, g, d, frames=16, camera=cams[1], flags=HH_FLAG_LIGHT_POWER)
+    crop = (200, 100, 248, 148)
+    x0, y0, x1, y1 = crop
+    img_o = o.render(camera=cams[1], frames=16, crop=crop, flags=HH_FLAG_LIGHT_POWER)
+    assert relmse(img_g[y0:y1, x0:x1], img_o[y0:y1, x0:x1]) < 1e-4
+    # a different estimator from the reference's area lights
+    img_r = o.render(camera=cams[1], frames=16, crop=crop)
+    assert relmse(img_r[y0:y1, x0:x1], img_o[y0:y1, x0:x1]) > 1e-6
