@@ -481,14 +481,19 @@ def main():
         per_launch = alg_bytes / max(cprof["extend"]["launches"], 1)
         avg_ms = ext["ms"] / max(ext["launches"], 1)
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        traffic = None
+        traffic, l2 = None, None
         tfile = ROOT / "profiles" / "extend_traffic.json"
         if tfile.exists():
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+            tj = json.loads(tfile.read_text())
+            traffic = tj.get("hbm_bytes_per_launch")
+            l2 = tj.get("l2")
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
             "kernel": "k_extend (closest-hit traversal)",
+            "l2_read_gbps": l2.get("l2_read_gbps") if l2 else None,
+            "l1_hit_rate": l2.get("l1_hit_rate") if l2 else None,
+            "l2_hit_rate": l2.get("l2_hit_rate") if l2 else None,
             "alg_bytes_per_launch": round(per_launch),
             "avg_launch_ms": round(avg_ms, 4),
             "per_ray": {"nodes": round(cnt["ext_nodes"] / rays, 2), "tris": round(cnt["ext_tris"] / rays, 2),

@@ -94,6 +94,37 @@ def mfma_md(stats_csv, pmc_csv, cmd, peak_tflops=2500.0, simds=1024):
     return "\n".join(lines) + "\n"
 
 
+def l2_traffic(pmc_csv, kernel, avg_ms, cmd):
+    """L1 -> L2 read traffic of a kernel: TCP_TCC_READ_REQ per dispatch, in
+    bytes by what the same counter reads for the 1 GiB copy kernel (128 B per
+    request on gfx950), plus the L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS))
+    and the mean L1 -> L2 read latency in cycles."""
+    c = per_dispatch_multi(pmc_csv)
+    probe = c["k_copy_probe"]
+    req_bytes = PROBE_BYTES / (sum(probe["TCP_TCC_READ_REQ_sum"]) / len(probe["TCP_TCC_READ_REQ_sum"]))
+    k = [v for n, v in c.items() if kernel in n and "true" not in n][0]
+    avg = {name: sum(vs) / len(vs) for name, vs in k.items()}
+    rd = avg["TCP_TCC_READ_REQ_sum"] * req_bytes
+    return {
+        "command": cmd, "bytes_per_request": round(req_bytes, 1),
+        "l1_accesses_per_launch": round(avg["TCP_TOTAL_CACHE_ACCESSES_sum"]),
+        "l2_read_bytes_per_launch": round(rd),
+        "l2_read_gbps": round(rd / (avg_ms * 1e-3) / 1e9, 1),
+        "l1_hit_rate": round(1.0 - avg["TCP_TCC_READ_REQ_sum"] / avg["TCP_TOTAL_CACHE_ACCESSES_sum"], 4),
+        "l2_hit_rate": round(avg["TCC_HIT_sum"] / (avg["TCC_HIT_sum"] + avg["TCC_MISS_sum"]), 4),
+        "l1_to_l2_read_latency_cycles": round(avg["TCP_TCC_READ_REQ_LATENCY_sum"] / avg["TCP_TCC_READ_REQ_sum"], 1),
+    }
+
+
+def per_dispatch_multi(csv_path):
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(csv_path)):
+        n = short(r["Kernel_Name"])
+        agg[n.split("<")[0].split("::")[-1] if "k_copy_probe" in n else n][r["Counter_Name"]].append(
+            float(r["Counter_Value"]))
+    return agg
+
+
 def traffic(fetch_csv, write_csv, kernel, cmd):
     f, w = per_dispatch(fetch_csv), per_dispatch(write_csv)
     probe_f = [v for k, vs in f.items() if "k_copy_probe" in k for v in vs]
@@ -130,6 +161,9 @@ def main():
     t.add_argument("--kernel", default="k_extend")
     t.add_argument("--cmd", required=True)
     t.add_argument("--out", required=True)
+    t.add_argument("--l2-csv", default=None, help="counter run with TCP_TCC_READ_REQ_sum, TCC_HIT_sum, ...")
+    t.add_argument("--l2-cmd", default="")
+    t.add_argument("--avg-ms", type=float, default=0.0, help="the kernel's average launch time")
     m = sub.add_parser("mfma")
     m.add_argument("stats_csv")
     m.add_argument("pmc_csv")
@@ -142,7 +176,10 @@ def main():
     elif a.what == "mfma":
         Path(a.out).write_text(mfma_md(a.stats_csv, a.pmc_csv, a.cmd))
     else:
-        Path(a.out).write_text(json.dumps(traffic(a.fetch_csv, a.write_csv, a.kernel, a.cmd), indent=1) + "\n")
+        res = traffic(a.fetch_csv, a.write_csv, a.kernel, a.cmd)
+        if a.l2_csv:
+            res["l2"] = l2_traffic(a.l2_csv, a.kernel, a.avg_ms, a.l2_cmd)
+        Path(a.out).write_text(json.dumps(res, indent=1) + "\n")
     print("wrote", a.out)
 
 
