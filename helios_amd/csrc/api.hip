@@ -1191,6 +1191,24 @@ hh_status hh_surface_write(hh_surface s, const float* in) {
     return HH_OK;
 }
 
+hh_status hh_surface_validate(hh_surface s, uint64_t* nonfinite) {
+    HH_ARG(s, "null surface");
+    hh_context ctx = s->ctx;
+    HH_HIP(hipSetDevice(ctx->device));
+    unsigned long long* d = nullptr;
+    HH_HIP(hipMalloc((void**)&d, sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = launch_count_nonfinite(ctx->stream, s->d, size_t(s->w) * s->h, d);
+    unsigned long long n = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, d, sizeof(n), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "surface validation");
+    if (nonfinite) *nonfinite = n;
+    if (n) return fail(HH_ERR_NAN_DETECTED, "%llu non-finite values in the surface", n);
+    return HH_OK;
+}
+
 hh_status hh_surface_device_ptr(hh_surface s, void** out) {
     HH_ARG(s && out, "null argument");
     *out = s->d;

@@ -1034,6 +1034,23 @@ hipError_t launch_sky_bake_equirect(hipStream_t st, uint16_t* texels, uint32_t w
     return hipGetLastError();
 }
 
+// Counts non-finite components of a float4 image (one atomic per wave).
+__global__ void k_count_nonfinite(const float4* __restrict__ p, size_t n, unsigned long long* __restrict__ out) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = p[i];
+        c += !isfinite(v.x) + !isfinite(v.y) + !isfinite(v.z) + !isfinite(v.w);
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63u) == 0) atomicAdd(out, c);
+}
+
+hipError_t launch_count_nonfinite(hipStream_t st, const float4* p, size_t n, unsigned long long* out) {
+    const uint32_t blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 2048));
+    k_count_nonfinite<<<blocks, 256, 0, st>>>(p, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f4(hipStream_t st, float4* p, float4 v, uint32_t n) {
     if (n == 0) return hipSuccess;
     k_fill_f4<<<cdiv(n, 256), 256, 0, st>>>(p, v, n);

@@ -132,6 +132,7 @@ hipError_t launch_sky_bake_equirect(hipStream_t st, uint16_t* texels, uint32_t w
 // Host Hosek-Wilkie fit (sky_host.cpp)
 void sky_coefficients(float turbidity, float albedo, float normalized_sun_y, const float sun_dir[3], float out[30]);
 hipError_t launch_fill_f4(hipStream_t st, float4* p, float4 v, uint32_t n);
+hipError_t launch_count_nonfinite(hipStream_t st, const float4* p, size_t n, unsigned long long* out);
 
 // The learned denoiser (denoise.hip): U-Net of 14 3x3 convolutions + a 1x1
 // head, BF16 MFMA kernels with FP32 accumulation, Adam on FP32 master weights.

@@ -64,6 +64,7 @@ _SIGS = {
     "hh_surface_write": [C.c_void_p, C.c_void_p],
     "hh_surface_device_ptr": [C.c_void_p, C.POINTER(C.c_void_p)],
     "hh_surface_enable_features": [C.c_void_p],
+    "hh_surface_validate": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hh_surface_read_features": [C.c_void_p, C.c_void_p],
     "hh_tonemap": [C.c_void_p, C.c_float, C.c_uint32, C.c_void_p],
     "hh_integrator_create": [C.c_void_p, C.POINTER(C.c_void_p)],
@@ -303,6 +304,12 @@ class Surface:
     def write(self, img):
         img = np.ascontiguousarray(img, np.float32)
         check(lib().hh_surface_write(self.h, ptr(img)))
+
+    def validate(self):
+        """Number of non-finite components (raises HH_ERR_NAN_DETECTED when any)."""
+        n = C.c_uint64()
+        check(lib().hh_surface_validate(self.h, C.byref(n)))
+        return n.value
 
     def enable_features(self):
         check(lib().hh_surface_enable_features(self.h))

@@ -288,6 +288,8 @@ hh_status hh_surface_clear(hh_surface s);  /* vkCmdClearColorImage to (0,0,0,1) 
 hh_status hh_surface_read(hh_surface s, float* out_rgba);
 hh_status hh_surface_write(hh_surface s, const float* rgba);
 hh_status hh_surface_device_ptr(hh_surface s, void** out);
+/* Counts non-finite components of the surface; HH_ERR_NAN_DETECTED when any. */
+hh_status hh_surface_validate(hh_surface s, uint64_t* nonfinite);
 /* First-hit feature planes for the denoiser (no counterpart in the
  * reference). Once enabled, every progressive frame also accumulates, with
  * the radiance's own recurrence, plane 0 = (albedo rgb, 1/(1+t)) and plane 1

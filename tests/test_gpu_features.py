@@ -101,3 +101,22 @@ def test_megakernel_writes_the_same_feature_planes(gpu_ctx):
         planes.append(surf.read_features())
     for a, b in zip(*planes):
         np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
+
+
+def test_surface_validate_counts_nonfinite(gpu_ctx):
+    from helios_amd import runtime
+    from helios_amd.runtime import HeliosError
+    d = scenegen.cornell_box()
+    scene = runtime.Scene(gpu_ctx, d)
+    surf = runtime.Surface(gpu_ctx, 32, 24)
+    it = runtime.PathIntegrator(gpu_ctx)
+    it.render(scene, d.camera, surf, frames=2)
+    assert surf.validate() == 0
+    img = surf.read()
+    img[3, 5, 1] = np.nan
+    img[7, 2, 0] = np.inf
+    img[0, 0, 3] = -np.inf
+    surf.write(img)
+    with pytest.raises(HeliosError) as e:
+        surf.validate()
+    assert e.value.status == 5 and "3 non-finite" in str(e.value)
