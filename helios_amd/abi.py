@@ -104,6 +104,18 @@ HH_FLAG_DIRECT_ONLY = 0x2
 HH_FLAG_MEGAKERNEL = 0x4
 HH_FLAG_LIGHT_POWER = 0x8
 
+# Output-buffer views (Renderer::OutputBuffer, renderer.h:27-35)
+OUTPUT_BUFFER_ALBEDO = 0
+OUTPUT_BUFFER_NORMALS = 1
+OUTPUT_BUFFER_ROUGHNESS = 2
+OUTPUT_BUFFER_METALLIC = 3
+OUTPUT_BUFFER_EMISSIVE = 4
+OUTPUT_BUFFER_FINAL = 5
+
+# DebugRayVertex (common.glsl:54-58); MAX_DEBUG_RAY_DRAW_COUNT (renderer.h:9)
+DEBUG_RAY_VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("color", "<f4", 4)])
+HH_MAX_DEBUG_RAY_DRAW_COUNT = 1024
+
 TEXTURE_RGBA8_UNORM = 0
 TEXTURE_RGBA8_SRGB = 1
 TEXTURE_RGBA32F = 2

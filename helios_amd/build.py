@@ -23,6 +23,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "bvh_build.hip", "integrator.hip", "sky_host.hip", "denoise.hip"]
+HOST_SOURCES = ["image_io.cpp"]    # host-only C++ (g++), linked with zlib
+CXX = os.environ.get("CXX", "g++")
 
 
 def flags():
@@ -35,7 +37,11 @@ def flags():
 
 def compile_one(src):
     obj = BUILD / (Path(src).stem + ".o")
-    cmd = [HIPCC, *flags(), "-c", str(CSRC / src), "-o", str(obj)]
+    if src in HOST_SOURCES:
+        cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{PKG.parent / 'include'}", "-c", str(CSRC / src),
+               "-o", str(obj)]
+    else:
+        cmd = [HIPCC, *flags(), "-c", str(CSRC / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -44,10 +50,11 @@ def compile_one(src):
 
 def build(verbose=True):
     BUILD.mkdir(exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+    srcs = SOURCES + HOST_SOURCES
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(compile_one, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

@@ -41,6 +41,9 @@ static hh_status fail(hh_status s, const char* fmt, ...) {
     g_err = buf;
     return s;
 }
+// Lets the host-only translation units (image and scene files) set the
+// thread's error string.
+hh_status hh_internal_fail(hh_status s, const char* msg) { return fail(s, "%s", msg); }
 static hh_status hip_fail(hipError_t e, const char* what) {
     (void)hipGetLastError();
     return fail(e == hipErrorOutOfMemory ? HH_ERR_OUT_OF_MEMORY : HH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
@@ -1735,6 +1738,142 @@ hh_status hh_primary_hits(hh_scene s, const hh_camera* cam, uint32_t w, uint32_t
     HH_HIP(hipStreamSynchronize(ctx->stream));
     to_hits(hit, ids, out, n);
     return HH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Output-buffer views and the ray-debug variant
+// ---------------------------------------------------------------------------
+static hipError_t ensure_spill_keep_graphs(hh_context ctx, size_t lanes) {
+    uint32_t* before = ctx->wf.spill;
+    HH_TRY(ctx->wf.ensure_spill(lanes));
+    if (ctx->wf.spill != before) ctx->drop_graphs();  // captured frames hold the old pointer
+    return hipSuccess;
+}
+
+hh_status hh_render_output_buffer(hh_scene s, const hh_camera* cam, uint32_t which, hh_surface out) {
+    HH_ARG(cam && out, "null argument");
+    HH_ARG(which <= HH_OUTPUT_BUFFER_FINAL, "unknown output buffer");
+    HH_ARG(which != HH_OUTPUT_BUFFER_FINAL, "HH_OUTPUT_BUFFER_FINAL is the path-traced surface, not a view");
+    HH_ARG(cam->near_plane > 0.0f && cam->far_plane > cam->near_plane, "camera needs 0 < near_plane < far_plane");
+    hh_status st = require_committed(s);
+    if (st != HH_OK) return st;
+    hh_context ctx = s->ctx;
+    HH_ARG(out->ctx == ctx, "surface belongs to another context");
+    HH_HIP(hipSetDevice(ctx->device));
+    const uint32_t w = out->w, h = out->h;
+    const size_t n = size_t(w) * h;
+    HH_HIP(ensure_dbg(ctx, sizeof(ViewGPU)));
+    ViewGPU* d_view = (ViewGPU*)ctx->dbg;
+    ViewGPU v = make_view(*cam, w, h);
+    HH_HIP(hipMemcpyAsync(d_view, &v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
+    HH_HIP(ensure_spill_keep_graphs(ctx, (n + 255) / 256 * 256));
+    const double fn = cam->far_plane, nn = cam->near_plane;
+    const float near_d = (float)(2.0 * fn * nn / (fn + nn));
+    LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
+    HH_HIP(launch_output_buffer(cfg, s->d_gpu, s->gpu, d_view, w, h, near_d, cam->far_plane, which, out->d, ctx->wf.spill));
+    // the view record was read from a stack variable
+    HH_HIP(hipStreamSynchronize(ctx->stream));
+    return HH_OK;
+}
+
+struct hh_debug_rays_t {
+    hh_context ctx = nullptr;
+    float4* vtx = nullptr;        // 2 float4 per vertex: position, colour
+    uint32_t* args = nullptr;     // {vertex count, instance count, first vertex, first instance}
+    uint32_t cap = 0;             // vertices
+    uint32_t reset_args[4] = {0, 1, 0, 0};
+    ~hh_debug_rays_t() {
+        if (vtx) (void)hipFree(vtx);
+        if (args) (void)hipFree(args);
+    }
+};
+
+hh_status hh_debug_rays_create(hh_context ctx, uint32_t max_segments, hh_debug_rays* out) {
+    HH_ARG(ctx && out && max_segments > 0 && max_segments <= (1u << 24), "bad argument");
+    HH_HIP(hipSetDevice(ctx->device));
+    hh_debug_rays r = new (std::nothrow) hh_debug_rays_t();
+    if (!r) return fail(HH_ERR_OUT_OF_MEMORY, "debug ray buffer");
+    r->ctx = ctx;
+    r->cap = 2 * max_segments;
+    hipError_t e = hipMalloc(&r->vtx, size_t(r->cap) * 32);
+    if (e == hipSuccess) e = hipMalloc(&r->args, 16);
+    if (e == hipSuccess) e = hipMemcpy(r->args, r->reset_args, 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete r;
+        return hip_fail(e, "hh_debug_rays_create");
+    }
+    *out = r;
+    return HH_OK;
+}
+
+hh_status hh_debug_rays_destroy(hh_debug_rays r) {
+    if (!r) return HH_OK;
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipStreamSynchronize(r->ctx->stream);
+    delete r;
+    return HH_OK;
+}
+
+hh_status hh_debug_rays_clear(hh_debug_rays r) {  // renderer.cpp:234-246
+    HH_ARG(r, "null argument");
+    HH_HIP(hipSetDevice(r->ctx->device));
+    HH_HIP(hipMemcpyAsync(r->args, r->reset_args, 16, hipMemcpyHostToDevice, r->ctx->stream));
+    return HH_OK;
+}
+
+hh_status hh_integrator_gather_debug_rays(hh_integrator it, hh_scene s, const hh_camera* cam, uint32_t width,
+                                          uint32_t height, uint32_t pixel_x, uint32_t pixel_y, uint32_t num_debug_rays,
+                                          hh_debug_rays out) {
+    HH_ARG(it && cam && out && width > 0 && height > 0 && width <= 16384 && height <= 16384, "bad argument");
+    HH_ARG(pixel_x < width && pixel_y <= height, "pixel outside the view");
+    HH_ARG(num_debug_rays <= (1u << 20), "too many debug rays");
+    hh_status st = require_committed(s);
+    if (st != HH_OK) return st;
+    hh_context ctx = s->ctx;
+    HH_ARG(it->ctx == ctx && out->ctx == ctx, "objects belong to different contexts");
+    HH_HIP(hipSetDevice(ctx->device));
+    HH_HIP(ensure_dbg(ctx, sizeof(ViewGPU)));
+    ViewGPU* d_view = (ViewGPU*)ctx->dbg;
+    ViewGPU v = make_view(*cam, width, height);
+    v.max_bounces = it->s.max_ray_bounces;
+    v.shadow_bias = it->s.shadow_ray_bias;
+    v.flags = it->s.flags & HH_FLAG_LIGHT_POWER;
+    HH_HIP(hipMemcpyAsync(d_view, &v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
+    HH_HIP(ensure_spill_keep_graphs(ctx, (size_t(num_debug_rays) + 63) / 64 * 64));
+    LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
+    // path_integrator.cpp:146: ray_debug_pixel_coord = (x, height - y, width, height)
+    HH_HIP(launch_ray_debug(cfg, s->d_gpu, s->gpu, d_view, num_debug_rays, pixel_x, height - pixel_y, it->num_acc, out->vtx,
+                            out->cap, out->args, ctx->wf.spill));
+    HH_HIP(hipStreamSynchronize(ctx->stream));
+    return HH_OK;
+}
+
+hh_status hh_debug_rays_read(hh_debug_rays r, hh_debug_ray_vertex* out, uint32_t max_vertices, uint32_t draw_args[4]) {
+    HH_ARG(r && draw_args && (out || max_vertices == 0), "bad argument");
+    HH_HIP(hipSetDevice(r->ctx->device));
+    HH_HIP(hipMemcpyAsync(draw_args, r->args, 16, hipMemcpyDeviceToHost, r->ctx->stream));
+    HH_HIP(hipStreamSynchronize(r->ctx->stream));
+    const uint32_t n = std::min(std::min(draw_args[0], r->cap), max_vertices);
+    if (n) {
+        HH_HIP(hipMemcpyAsync(out, r->vtx, size_t(n) * 32, hipMemcpyDeviceToHost, r->ctx->stream));
+        HH_HIP(hipStreamSynchronize(r->ctx->stream));
+    }
+    return HH_OK;
+}
+
+hh_status hh_debug_rays_device_ptr(hh_debug_rays r, void** vertices, void** draw_args) {
+    HH_ARG(r && vertices && draw_args, "null argument");
+    *vertices = r->vtx;
+    *draw_args = r->args;
+    return HH_OK;
+}
+
+hh_status hh_surface_save_png(hh_surface s, float exposure, uint32_t op, const char* path) {
+    HH_ARG(s && path, "null argument");
+    std::vector<uint8_t> px(size_t(s->w) * s->h * 4);
+    hh_status st = hh_tonemap(s, exposure, op, px.data());
+    if (st != HH_OK) return st;
+    return hh_write_png(path, px.data(), s->w, s->h, 1u);
 }
 
 // ---------------------------------------------------------------------------

@@ -795,6 +795,161 @@ __global__ void __launch_bounds__(256) k_primary(const SceneGPU* __restrict__ sc
     out_ids[2 * i + 1] = prim;
 }
 
+// Output-buffer views (debug_visualization.vert:49-60, .frag:60-163). The
+// reference rasterises every submesh; a fragment's perspective-correct
+// attributes are the barycentric interpolation at the point where the ray
+// through the fragment centre meets the triangle, so one un-jittered ray per
+// pixel reproduces the pass: depth test LESS_OR_EQUAL = closest hit, cull
+// none, no any-hit (the pass draws alpha-tested geometry solid), and the clip
+// volume 0 <= z <= w of glm's GL-style projection = view depth in
+// [2fn/(f+n), f] (near_d, far_d).
+__device__ inline float4 output_buffer_value(const SceneGPU& S, uint32_t inst, uint32_t tri, float u, float v, uint32_t which) {
+    const float4* tw = S.tris + 3 * (size_t)tri;
+    uint32_t prim = __float_as_uint(tw[0].w);
+    uint32_t geom = __float_as_uint(tw[1].w) & 0x7FFFFFFFu;
+    const InstanceData& I = S.inst[inst];
+    uint32_t mesh = S.inst_trav[inst].mesh;
+    uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
+    const MaterialData& m = S.mats[info.y];
+    const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * (prim + info.x);
+    const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
+    const float4* Vk[3] = {vb + 5 * (size_t)ix[0], vb + 5 * (size_t)ix[1], vb + 5 * (size_t)ix[2]};
+    const float b[3] = {1.0f - u - v, u, v};
+    float uv0 = 0.f, uv1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        uv0 += Vk[k][1].x * b[k];
+        uv1 += Vk[k][1].y * b[k];
+    }
+    f3 c = mk(0.f, 0.f, 0.f);
+    if (which == 0) {  // fetch_albedo: material.albedo.rgb (Q10-converted) or texture rgb
+        if (m.tex0[0] == -1) c = mk(m.albedo[0], m.albedo[1], m.albedo[2]);
+        else c = ld3(sample_tex(S, m.tex0[0], uv0, uv1));
+    } else if (which == 1) {
+        // vertex stage: normalize(mat3(normal_matrix) * N) per vertex, then
+        // interpolated without renormalising (fetch_normal without a map)
+        f3 n = mk(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 3; k++) n = n + normalize(m3_mul(I.normal, ld3(Vk[k][2]))) * b[k];
+        if (m.tex0[1] != -1) {  // get_normal_from_map
+            f3 tg = mk(0.f, 0.f, 0.f), bt = mk(0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                tg = tg + normalize(m3_mul(I.normal, ld3(Vk[k][3]))) * b[k];
+                bt = bt + normalize(m3_mul(I.normal, ld3(Vk[k][4]))) * b[k];
+            }
+            f3 T = normalize(tg), B = normalize(bt), N = normalize(n);
+            float4 t = sample_tex(S, m.tex0[1], uv0, uv1);
+            f3 tn = normalize(mk(t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f));
+            n = normalize(T * tn.x + B * tn.y + N * tn.z);
+        }
+        c = n * 0.5f + mk(0.5f, 0.5f, 0.5f);
+    } else if (which == 2 || which == 3) {  // fetch_roughness / fetch_metallic (no MIN_ROUGHNESS clamp here)
+        const int k = which - 2;
+        float r;
+        if (m.tex0[2 + k] == -1) r = m.roughness_metallic[k];
+        else {
+            float4 t = sample_tex(S, m.tex0[2 + k], uv0, uv1);
+            const int ch = m.tex1[2 + k] & 3;
+            r = ch == 0 ? t.x : (ch == 1 ? t.y : (ch == 2 ? t.z : t.w));
+        }
+        c = mk(r, r, r);
+    } else {  // fetch_emissive
+        if (m.tex1[0] == -1) c = mk(m.emissive[0], m.emissive[1], m.emissive[2]);
+        else c = ld3(sample_tex(S, m.tex1[0], uv0, uv1));
+    }
+    return make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) k_output_buffer(const SceneGPU* __restrict__ scene, const ViewGPU* __restrict__ view,
+                                                       uint32_t w, uint32_t h_, float near_d, float far_d, uint32_t which,
+                                                       float4* __restrict__ out, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t stack[kLdsStack * 256];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), 256, to_global(spill + i), gridDim.x * 256};
+    if (i >= w * h_) return;
+    const uint32_t x = i % w, y = i / w;
+    const ViewGPU& V = *view;
+    const SceneGPU& S = *scene;
+    const float nx = (((float)x + 0.5f) / (float)w) * 2.0f - 1.0f;
+    const float ny = (((float)y + 0.5f) / (float)h_) * 2.0f - 1.0f;
+    float tg[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) tg[r] = (V.vpi[0 * 4 + r] * nx + V.vpi[1 * 4 + r] * ny) + V.vpi[3 * 4 + r];
+    const f3 cam = mk(V.cam[0], V.cam[1], V.cam[2]);
+    const f3 d = normalize(mk(tg[0] / tg[3], tg[1] / tg[3], tg[2] / tg[3]) - cam);
+    // the focal plane's normal is -forward
+    const float cosf_ = -(d.x * V.fplane[0] + d.y * V.fplane[1] + d.z * V.fplane[2]);
+    float4 res = make_float4(0.f, 0.f, 0.f, 1.f);  // clear colour (renderer.cpp:282-287)
+    if (cosf_ > 0.0f) {
+        HitRec hr;
+        trace_ray<W, false>(S, cam, d, near_d / cosf_, far_d / cosf_, HH_TRACE_OPAQUE_BIT, hr, stk);
+        if (hr.found) res = output_buffer_value(S, hr.inst, hr.tri, hr.u, hr.v, which);
+    }
+    out[i] = res;
+}
+
+// Ray-debug path variant (path_trace_debug.rgen, RAY_DEBUG_VIEW in
+// path_trace_rgen.glsl:190-195, path_trace_rchit.glsl:500-514,548-567 and
+// path_trace_rmiss.glsl:40-58): one path per debug ray through one pixel,
+// launch id (i, 0) and seed num_frames; a colour from three draws before the
+// camera ray; no Russian roulette (and no RR draw); every non-primary ray
+// appends the segment origin -> hit (or origin + dir * tmax on a miss) as two
+// DebugRayVertex records. Shadow rays only set visibility, which no segment
+// depends on, so they are not traced; their draws are still consumed.
+// vtx holds cap vertices; args[0] counts every appended vertex, also those
+// that did not fit.
+template <int W>
+__global__ void __launch_bounds__(64) k_ray_debug(const SceneGPU* __restrict__ scene, const ViewGPU* __restrict__ view,
+                                                  uint32_t n, uint32_t px, uint32_t py, uint32_t frame,
+                                                  float4* __restrict__ vtx, uint32_t cap, uint32_t* __restrict__ args,
+                                                  uint32_t* __restrict__ spill) {
+    __shared__ uint32_t stack[kLdsStack * 64];
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    const LaneStack stk{to_lds(&stack[threadIdx.x]), 64, to_global(spill + i), gridDim.x * 64};
+    if (i >= n) return;
+    const ViewGPU& V = *view;
+    const SceneGPU& S = *scene;
+    RNG rng = rng_init(i, 0u, frame);
+    const float cr = next_float(rng) * 0.5f + 0.5f;
+    const float cg = next_float(rng) * 0.5f + 0.5f;
+    const float cb = next_float(rng) * 0.5f + 0.5f;
+    f3 o, d;
+    generate_ray(V, px, py, rng, o, d);
+    f3 T = mk(1.f, 1.f, 1.f);
+    float tmin = 0.001f;
+    uint32_t rflags = 0;
+    for (uint32_t depth = 0;; depth++) {
+        HitRec h;
+        trace_ray<W, false>(S, o, d, tmin, 10000.0f, rflags, h, stk);
+        if (depth > 0) {
+            const f3 e = o + d * (h.found ? h.t : 10000.0f);
+            const uint32_t k = atomicAdd(&args[0], 2u);
+            if (k + 2u <= cap) {
+                vtx[2 * k + 0] = make_float4(o.x, o.y, o.z, 1.0f);
+                vtx[2 * k + 1] = make_float4(cr, cg, cb, 1.0f);
+                vtx[2 * k + 2] = make_float4(e.x, e.y, e.z, 1.0f);
+                vtx[2 * k + 3] = make_float4(cr, cg, cb, 1.0f);
+            }
+        }
+        if (!h.found) break;
+        Surface p;
+        populate(S, h.inst, h.tri, h.u, h.v, p);
+        (void)direct_lighting(S, p, d, T, depth, V.shadow_bias, V.flags, rng);
+        if (depth + 1 >= V.max_bounces) break;
+        f3 Wi;
+        float pdf;
+        f3 brdf = sample_uber(p, -d, rng, Wi, pdf);
+        float ct = clampf(dot(p.normal, Wi), 0.0f, 1.0f);
+        T = (T * (brdf * ct)) / pdf;
+        o = p.pos;
+        d = Wi;
+        tmin = 0.0001f;
+        rflags = HH_TRACE_OPAQUE_BIT;
+    }
+}
+
 // tone_map.frag:20-51
 __global__ void k_tonemap(const float4* __restrict__ src, uint32_t n, float exposure, uint32_t op, uchar4* __restrict__ dst) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1017,7 +1172,29 @@ hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, co
     return hipGetLastError();
 }
 
-hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst) {
+hipError_t launch_output_buffer(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
+                                uint32_t w, uint32_t h, float near_d, float far_d, uint32_t which, float4* out,
+                                uint32_t* spill) {
+    const uint32_t n = w * h;
+    const int L = h_scene.bvh_width;
+    if (L == kLayoutB8) k_output_buffer<kLayoutB8><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, near_d, far_d, which, out, spill);
+    else if (L == kLayoutQ4) k_output_buffer<kLayoutQ4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, near_d, far_d, which, out, spill);
+    else k_output_buffer<kLayoutB4><<<cdiv(n, 256), 256, 0, cfg.stream>>>(d_scene, d_view, w, h, near_d, far_d, which, out, spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_ray_debug(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
+                            uint32_t n, uint32_t px, uint32_t py, uint32_t frame, float4* vtx, uint32_t cap, uint32_t* args,
+                            uint32_t* spill) {
+    if (n == 0) return hipSuccess;
+    const int L = h_scene.bvh_width;
+    if (L == kLayoutB8) k_ray_debug<kLayoutB8><<<cdiv(n, 64), 64, 0, cfg.stream>>>(d_scene, d_view, n, px, py, frame, vtx, cap, args, spill);
+    else if (L == kLayoutQ4) k_ray_debug<kLayoutQ4><<<cdiv(n, 64), 64, 0, cfg.stream>>>(d_scene, d_view, n, px, py, frame, vtx, cap, args, spill);
+    else k_ray_debug<kLayoutB4><<<cdiv(n, 64), 64, 0, cfg.stream>>>(d_scene, d_view, n, px, py, frame, vtx, cap, args, spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_tonemap(hipStream_t st,const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst) {
     k_tonemap<<<cdiv(n, 256), 256, 0, st>>>(src, n, exposure, op, dst);
     return hipGetLastError();
 }

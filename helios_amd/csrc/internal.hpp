@@ -125,6 +125,11 @@ hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, cons
                              uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
 hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
                                uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
+hipError_t launch_output_buffer(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
+                                uint32_t w, uint32_t h, float near_d, float far_d, uint32_t which, float4* out, uint32_t* spill);
+hipError_t launch_ray_debug(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
+                            uint32_t n, uint32_t px, uint32_t py, uint32_t frame, float4* vtx, uint32_t cap, uint32_t* args,
+                            uint32_t* spill);
 hipError_t launch_tonemap(hipStream_t st, const float4* src, uint32_t n, float exposure, uint32_t op, uchar4* dst);
 hipError_t launch_sky_bake(hipStream_t st, float4* faces, uint32_t size, const float* d_coef, const float sun[3]);
 hipError_t launch_sky_bake_equirect(hipStream_t st, uint16_t* texels, uint32_t width, const float* d_coef, const float sun[3]);

@@ -88,6 +88,16 @@ _SIGS = {
     "hh_context_read_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "hh_context_reset_profile": [C.c_void_p],
     "hh_bandwidth_probe": [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)],
+    "hh_render_output_buffer": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_void_p],
+    "hh_surface_save_png": [C.c_void_p, C.c_float, C.c_uint32, C.c_char_p],
+    "hh_write_png": [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32],
+    "hh_debug_rays_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)],
+    "hh_debug_rays_destroy": [C.c_void_p],
+    "hh_debug_rays_clear": [C.c_void_p],
+    "hh_debug_rays_read": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "hh_debug_rays_device_ptr": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    "hh_integrator_gather_debug_rays": [C.c_void_p, C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p],
 }
 
 KERNEL_KINDS = ("generate", "extend", "shade", "shadow", "accumulate", "megakernel")
@@ -331,6 +341,10 @@ class Surface:
         check(lib().hh_tonemap(self.h, exposure, operator, ptr(out)))
         return out
 
+    def save_png(self, path, exposure=1.0, operator=0):
+        """Renderer::save_image_to_disk: tone-mapped RGBA8 PNG, top row first."""
+        check(lib().hh_surface_save_png(self.h, exposure, operator, str(path).encode()))
+
     def close(self):
         if self.h:
             lib().hh_surface_destroy(self.h)
@@ -408,6 +422,12 @@ class PathIntegrator:
     def render(self, scene, camera, surface, frames=1):
         check(lib().hh_integrator_render_frames(self.h, scene.h, C.byref(camera_struct(camera)), surface.h, frames))
 
+    def gather_debug_rays(self, scene, camera, width, height, pixel, num_debug_rays, buffer):
+        """PathIntegrator::gather_debug_rays: append the segments of num_debug_rays
+        paths through window pixel (x, y) (rows top-down) to a DebugRays buffer."""
+        check(lib().hh_integrator_gather_debug_rays(self.h, scene.h, C.byref(camera_struct(camera)), width, height,
+                                                    int(pixel[0]), int(pixel[1]), num_debug_rays, buffer.h))
+
     def ray_stats(self):
         out = np.zeros(3, np.uint64)
         check(lib().hh_integrator_ray_stats(self.h, ptr(out)))
@@ -416,6 +436,49 @@ class PathIntegrator:
     def close(self):
         if self.h:
             lib().hh_integrator_destroy(self.h)
+            self.h = None
+
+
+def render_output_buffer(scene, camera, which, surface):
+    """Output-buffer view (abi.OUTPUT_BUFFER_*) of `scene` into `surface`."""
+    check(lib().hh_render_output_buffer(scene.h, C.byref(camera_struct(camera)), which, surface.h))
+
+
+def write_png(path, rgba8, flip_rows=False):
+    """Write an (H, W, 4) uint8 array as a PNG (row 0 first unless flip_rows)."""
+    a = np.ascontiguousarray(rgba8, np.uint8)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("expected an (H, W, 4) uint8 array")
+    check(lib().hh_write_png(str(path).encode(), ptr(a), a.shape[1], a.shape[0], int(bool(flip_rows))))
+
+
+class DebugRays:
+    """The ray-debug vertex buffer and its draw arguments (renderer.cpp:1473-1479)."""
+
+    def __init__(self, ctx, max_segments=abi.HH_MAX_DEBUG_RAY_DRAW_COUNT):
+        self.ctx = ctx
+        self.capacity = 2 * max_segments
+        self.h = C.c_void_p()
+        check(lib().hh_debug_rays_create(ctx.h, max_segments, C.byref(self.h)))
+
+    def clear(self):
+        check(lib().hh_debug_rays_clear(self.h))
+
+    def read(self):
+        """(vertices (n,) DEBUG_RAY_VERTEX_DTYPE, draw_args[4]); n = min(count, capacity)."""
+        args = np.zeros(4, np.uint32)
+        out = np.zeros(self.capacity, abi.DEBUG_RAY_VERTEX_DTYPE)
+        check(lib().hh_debug_rays_read(self.h, ptr(out), self.capacity, ptr(args)))
+        return out[:min(int(args[0]), self.capacity)], args
+
+    def device_ptrs(self):
+        v, a = C.c_void_p(), C.c_void_p()
+        check(lib().hh_debug_rays_device_ptr(self.h, C.byref(v), C.byref(a)))
+        return v.value, a.value
+
+    def close(self):
+        if self.h:
+            lib().hh_debug_rays_destroy(self.h)
             self.h = None
 
 

@@ -45,7 +45,8 @@ typedef enum hh_status {
     HH_ERR_LIMIT_EXCEEDED = 3,
     HH_ERR_DEVICE = 4,
     HH_ERR_NAN_DETECTED = 5,
-    HH_ERR_NOT_READY = 6
+    HH_ERR_NOT_READY = 6,
+    HH_ERR_IO = 7              /* a file could not be read, written or parsed */
 } hh_status;
 
 typedef struct hh_context_t* hh_context;
@@ -53,6 +54,7 @@ typedef struct hh_scene_t* hh_scene;
 typedef struct hh_surface_t* hh_surface;
 typedef struct hh_integrator_t* hh_integrator;
 typedef struct hh_denoiser_t* hh_denoiser;
+typedef struct hh_debug_rays_t* hh_debug_rays;
 
 /* ---------------------------------------------------------------------------
  * Data layouts shared with the reference (byte compatible).
@@ -302,6 +304,39 @@ hh_status hh_surface_read_features(hh_surface s, float* out);
 /* tone_map.frag: exposure -> ACES (op 0) / Reinhard (op 1) / none -> gamma
  * 1/2.2 -> RGBA8 (row 0 = bottom, as stored). */
 hh_status hh_tonemap(hh_surface s, float exposure, uint32_t op, uint8_t* out_rgba8);
+/* Renderer::save_image_to_disk + copy_and_save_tone_mapped_image
+ * (renderer.cpp:637-713, 755-768): hh_tonemap, then an 8-bit RGBA PNG with
+ * the top image row first (the reference's tone-map pass flips the stored
+ * rows with a negative viewport, renderer.cpp:396-403). */
+hh_status hh_surface_save_png(hh_surface s, float exposure, uint32_t op, const char* path);
+/* The PNG writer alone (stbi_write_png's role): RGBA8 rows as given, row 0
+ * first; flip_rows != 0 writes the last row first. */
+hh_status hh_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height, uint32_t flip_rows);
+
+/* Output-buffer views (Renderer::OutputBuffer, renderer.h:27-35;
+ * render_debug_visualization, renderer.cpp:491-547; debug_visualization.vert
+ * and .frag:60-163). Writes, per pixel of `out` (rows in the surface's order:
+ * pixel (x, y) sees what the path tracer's pixel (x, y) sees), the closest
+ * surface's
+ *   ALBEDO    material albedo rgb (as stored, Q10-converted) or texture rgb,
+ *   NORMALS   normal * 0.5 + 0.5, the normal being the interpolation of the
+ *             per-vertex world normals normalize(mat3(normal_matrix) * N)
+ *             (normal-mapped when the material has a map),
+ *   ROUGHNESS roughness (not clamped to the path's 0.1 minimum),
+ *   METALLIC  metallic,
+ *   EMISSIVE  emissive rgb,
+ * alpha 1, and (0, 0, 0, 1) where nothing is hit. Matches the raster pass:
+ * no alpha test, no culling, view depth clipped to [2fn/(f+n), f] (glm's
+ * GL-style projection under Vulkan's 0 <= z <= w clip volume), textures at
+ * level 0. FINAL is the path-traced surface itself and is rejected here.
+ * Asynchronous on the context stream. */
+#define HH_OUTPUT_BUFFER_ALBEDO 0u
+#define HH_OUTPUT_BUFFER_NORMALS 1u
+#define HH_OUTPUT_BUFFER_ROUGHNESS 2u
+#define HH_OUTPUT_BUFFER_METALLIC 3u
+#define HH_OUTPUT_BUFFER_EMISSIVE 4u
+#define HH_OUTPUT_BUFFER_FINAL 5u
+hh_status hh_render_output_buffer(hh_scene scene, const hh_camera* cam, uint32_t output_buffer, hh_surface out);
 
 /* ---------------------------------------------------------------------------
  * Integrator (replaces helios::PathIntegrator)
@@ -322,6 +357,42 @@ hh_status hh_integrator_counters(hh_integrator it, uint32_t out[4]);
 /* Path segments traced in the last render call: [extension rays incl.
  * primary, shadow rays, frames]. */
 hh_status hh_integrator_ray_stats(hh_integrator it, uint64_t out[3]);
+/* Ray-debug path variant (path_trace_debug.rgen; RAY_DEBUG_VIEW branches of
+ * path_trace_rgen.glsl:138-148,190-195, path_trace_rchit.glsl:500-514,548-567
+ * and path_trace_rmiss.glsl:40-58; PathIntegrator::gather_debug_rays,
+ * path_integrator.cpp:88-104; Renderer::add_ray_debug_view /
+ * clear_ray_debug_views and the vertex and draw-argument buffers,
+ * renderer.cpp:229-249,733-751,1473-1479). A segment buffer holds
+ * DebugRayVertex records (common.glsl:54-58) and VkDrawIndirectCommand-style
+ * draw arguments {vertex count, 1, 0, 0}; create and clear set them to
+ * {0, 1, 0, 0}. */
+typedef struct hh_debug_ray_vertex {
+    float position[4];
+    float color[4];
+} hh_debug_ray_vertex;
+#define HH_MAX_DEBUG_RAY_DRAW_COUNT 1024u   /* segments; renderer.h:9 */
+hh_status hh_debug_rays_create(hh_context ctx, uint32_t max_segments, hh_debug_rays* out);
+hh_status hh_debug_rays_destroy(hh_debug_rays r);
+hh_status hh_debug_rays_clear(hh_debug_rays r);
+/* Traces num_debug_rays paths through window pixel (pixel_x, pixel_y) of a
+ * width x height view (window rows top-down: the reference passes
+ * (pixel_x, height - pixel_y) to the shader), seeded with the integrator's
+ * current num_accumulated_samples and launch id (i, 0), max_ray_bounces and
+ * flags from its settings (HH_FLAG_LIGHT_POWER is honoured; the variant is
+ * always the full path integrator). Each path draws its colour
+ * (3 x (next_float * 0.5 + 0.5)) before the camera ray, skips Russian
+ * roulette, and appends one segment (2 vertices) per non-primary ray: origin
+ * to hit point, or origin + 10000 * direction on a miss. Appends after what
+ * the buffer holds, in no fixed order; the count keeps growing past the
+ * capacity, vertices beyond it are dropped. Asynchronous on the context
+ * stream. */
+hh_status hh_integrator_gather_debug_rays(hh_integrator it, hh_scene scene, const hh_camera* cam, uint32_t width,
+                                          uint32_t height, uint32_t pixel_x, uint32_t pixel_y, uint32_t num_debug_rays,
+                                          hh_debug_rays out);
+/* Copies min(count, capacity, max) vertices and the 4 draw arguments. */
+hh_status hh_debug_rays_read(hh_debug_rays r, hh_debug_ray_vertex* out, uint32_t max_vertices, uint32_t draw_args[4]);
+hh_status hh_debug_rays_device_ptr(hh_debug_rays r, void** vertices, void** draw_args);
+
 /* PushConstants as launch_rays fills them (path_integrator.cpp:136-159). */
 hh_status hh_compute_launch_consts(const hh_camera* cam, uint32_t width, uint32_t height,
                                    uint32_t num_lights, uint32_t num_frames,

@@ -67,6 +67,11 @@ def lib():
                                  C.c_int, vp, vp]
         L.orc_primary_hits.argtypes = [vp, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
         L.orc_trace_rays.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]
+        L.orc_output_buffer.argtypes = [vp, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
+        L.orc_output_buffer.restype = C.c_int
+        L.orc_ray_debug.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.Settings), C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_uint32]
+        L.orc_ray_debug.restype = C.c_uint32
         L.orc_rng_hash.argtypes = [C.c_uint32]
         L.orc_rng_hash.restype = C.c_uint32
         L.orc_rng_init.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, vp]
@@ -206,6 +211,32 @@ class OracleScene:
         lib().orc_primary_hits(self.h, C.byref(camera_struct(camera or d.camera)), width, height, frame, int(brute),
                                ptr(out))
         return out
+
+    def output_buffer(self, which, camera=None, width=None, height=None, brute=False):
+        """Output-buffer view `which` (0 albedo, 1 normals, 2 roughness, 3 metallic,
+        4 emissive) as an (height, width, 4) float32 image."""
+        d = self.desc
+        width = width or d.width
+        height = height or d.height
+        out = np.zeros((height, width, 4), np.float32)
+        if lib().orc_output_buffer(self.h, C.byref(camera_struct(camera or d.camera)), width, height, which,
+                                   int(brute), ptr(out)):
+            raise RuntimeError("oracle output_buffer failed")
+        return out
+
+    def ray_debug(self, pixel, num_rays, frame, camera=None, width=None, height=None, max_ray_bounces=None,
+                  flags=0):
+        """Ray-debug segments of num_rays paths through window pixel (x, y):
+        (n, 9) float32 = origin xyz, end xyz, colour rgb, ray by ray."""
+        d = self.desc
+        width = width or d.width
+        height = height or d.height
+        st = settings_struct(max_ray_bounces or d.max_ray_bounces, flags=flags)
+        cap = num_rays * max(st.max_ray_bounces, 1)
+        out = np.zeros((cap, 9), np.float32)
+        n = lib().orc_ray_debug(self.h, C.byref(camera_struct(camera or d.camera)), C.byref(st), width, height,
+                                pixel[0], pixel[1], num_rays, frame, ptr(out), cap)
+        return out[:min(n, cap)]
 
     def trace_rays(self, rays, flags=0, brute=False, fp64=False):
         rays = np.ascontiguousarray(rays)

@@ -1262,10 +1262,22 @@ struct Tracer {
         return L * R(nl);
     }
 
+    // RAY_DEBUG_VIEW variant (path_trace_rchit.glsl:500-514,548-567;
+    // path_trace_rmiss.glsl:40-58): no Russian roulette and no RR draw; every
+    // non-primary ray appends (origin, end, colour) to `segments`.
+    bool debug = false;
+    V3<R> debug_color;
+    std::vector<float>* segments = nullptr;
+
     // payload: L, T, depth, rng. Returns L of this ray.
     V3<R> trace_path(V3<R> o, V3<R> d, R tmin, uint32_t rflags, V3<R> T, uint32_t depth, RNG rng) {
         st.ext++;
         Hit<R> h = trace<R>(s, o, d, tmin, R(10000.0), rflags, true, brute);
+        if (debug && depth > 0) {
+            const V3<R> e = o + d * (h.found ? h.t : R(10000.0));
+            const R rec[9] = {o.x, o.y, o.z, e.x, e.y, e.z, debug_color.x, debug_color.y, debug_color.z};
+            for (R x : rec) segments->push_back((float)x);
+        }
         if (!h.found) {  // path_trace_rmiss.glsl:60-65
             V3<R> e = env(d);
             return depth == 0 ? e : T * e;
@@ -1285,6 +1297,7 @@ struct Tracer {
         V3<R> brdf = sample_uber(p, Wo, rng, Wi, pdf);
         R ct = gclamp(dot(p.normal, Wi), R(0), R(1));
         V3<R> Tn = (T * (brdf * ct)) / pdf;
+        if (debug) return trace_path(p.vertex.pos, Wi, R(0.0001), HH_TRACE_OPAQUE, Tn, depth + 1, rng);
         R prob = gmax(Tn.x, gmax(Tn.y, Tn.z));
         // the RR draw is consumed at every bounce so the stream stays aligned;
         // bounces before rr_start neither terminate nor rescale
@@ -1394,6 +1407,81 @@ static void render_impl(const Scene& s, const hh_camera& cam, const hh_settings&
         stats[0] = n_ext.load();
         stats[1] = n_sh.load();
     }
+}
+
+// Output-buffer views: debug_visualization.vert:49-60 and .frag:60-163,
+// rasterised by Renderer::render_debug_visualization (renderer.cpp:491-547;
+// pipeline state :1178-1300). Perspective-correct interpolation at a fragment
+// centre is barycentric interpolation at the point where the ray from the eye
+// through that centre meets the triangle, so each pixel is one un-jittered
+// ray: depth test LESS_OR_EQUAL = closest hit; VK_CULL_MODE_NONE; no discard,
+// so alpha-tested geometry is drawn solid (an opaque trace); glm's GL-style
+// projection under Vulkan's 0 <= z_clip <= w_clip keeps view depths in
+// [2fn/(f+n), f].
+template <typename R>
+static void output_buffer_value(const Scene& s, const Hit<R>& h, uint32_t which, R out[4]) {
+    const InstanceData& inst = s.inst_data[h.inst];
+    const uint32_t* info = &s.submesh_info[h.inst][h.geom * 2];
+    const Mesh& me = s.meshes[s.instances[h.inst].mesh];
+    const MaterialData& m = s.mat_data[info[1]];
+    const uint32_t pid = h.prim + info[0];
+    const hh_vertex* tv[3] = {&me.vertices[me.indices[3 * pid]], &me.vertices[me.indices[3 * pid + 1]],
+                              &me.vertices[me.indices[3 * pid + 2]]};
+    const R b[3] = {R(1) - h.u - h.v, h.u, h.v};
+    R uv0 = 0, uv1 = 0;
+    for (int k = 0; k < 3; k++) {
+        uv0 += R(tv[k]->tex_coord[0]) * b[k];
+        uv1 += R(tv[k]->tex_coord[1]) * b[k];
+    }
+    R c[4];
+    V3<R> v(0, 0, 0);
+    switch (which) {
+        case 0:  // fetch_albedo
+            if (m.tex0[0] == -1) v = v4xyz<R>(m.albedo);
+            else {
+                sample_texture<R>(s, m.tex0[0], uv0, uv1, c);
+                v = V3<R>(c[0], c[1], c[2]);
+            }
+            break;
+        case 1: {  // vertex stage normalises each transformed normal; no renormalisation after interpolation
+            V3<R> n(0, 0, 0), t(0, 0, 0), bt(0, 0, 0);
+            for (int k = 0; k < 3; k++) {
+                n = n + normalize(mat3_mul<R>(inst.normal, v4xyz<R>(tv[k]->normal))) * b[k];
+                t = t + normalize(mat3_mul<R>(inst.normal, v4xyz<R>(tv[k]->tangent))) * b[k];
+                bt = bt + normalize(mat3_mul<R>(inst.normal, v4xyz<R>(tv[k]->bitangent))) * b[k];
+            }
+            if (m.tex0[1] != -1) {  // get_normal_from_map
+                sample_texture<R>(s, m.tex0[1], uv0, uv1, c);
+                V3<R> T = normalize(t), B = normalize(bt), N = normalize(n);
+                V3<R> tn = normalize(V3<R>(c[0] * R(2) - R(1), c[1] * R(2) - R(1), c[2] * R(2) - R(1)));
+                n = normalize(T * tn.x + B * tn.y + N * tn.z);
+            }
+            v = n * R(0.5) + V3<R>(R(0.5), R(0.5), R(0.5));
+            break;
+        }
+        case 2:
+        case 3: {  // fetch_roughness / fetch_metallic (the raw value: no MIN_ROUGHNESS)
+            const int k = int(which) - 2;
+            R x;
+            if (m.tex0[2 + k] == -1) x = R(m.roughness_metallic[k]);
+            else {
+                sample_texture<R>(s, m.tex0[2 + k], uv0, uv1, c);
+                x = c[m.tex1[2 + k] & 3];
+            }
+            v = V3<R>(x, x, x);
+            break;
+        }
+        default:  // fetch_emissive
+            if (m.tex1[0] == -1) v = v4xyz<R>(m.emissive);
+            else {
+                sample_texture<R>(s, m.tex1[0], uv0, uv1, c);
+                v = V3<R>(c[0], c[1], c[2]);
+            }
+    }
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
+    out[3] = R(1);
 }
 
 }  // namespace orc
@@ -1541,6 +1629,70 @@ int orc_primary_hits(void* sp, const hh_camera* cam, uint32_t w, uint32_t h, uin
         }
     return 0;
 }
+// Output-buffer view `which` (0 albedo .. 4 emissive) into w*h RGBA floats,
+// rows in the path tracer's order; background (0, 0, 0, 1).
+int orc_output_buffer(void* sp, const hh_camera* cam, uint32_t w, uint32_t h, uint32_t which, int brute, float* out) {
+    Scene& s = *(Scene*)sp;
+    if (!s.committed || which > 4) return 1;
+    hh_settings st{};
+    hh_launch_consts c;
+    launch_consts(*cam, w, h, s.num_lights, 0, st, c);
+    const double fd = cam->far_plane, nd = cam->near_plane;
+    const float near_d = (float)(2.0 * fd * nd / (fd + nd)), far_d = cam->far_plane;
+    const V3<float> eye = v4xyz<float>(c.camera_pos);
+    const V3<float> fwd(-c.focal_plane[0], -c.focal_plane[1], -c.focal_plane[2]);
+    const float* m = c.view_proj_inverse;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            const float nx = ((float(x) + 0.5f) / float(w)) * 2.0f - 1.0f;
+            const float ny = ((float(y) + 0.5f) / float(h)) * 2.0f - 1.0f;
+            float tg[4];
+            for (int r = 0; r < 4; r++) tg[r] = (m[0 * 4 + r] * nx + m[1 * 4 + r] * ny) + m[3 * 4 + r];
+            const V3<float> d = normalize(V3<float>(tg[0] / tg[3], tg[1] / tg[3], tg[2] / tg[3]) - eye);
+            const float cs = dot(d, fwd);
+            float* px = out + (size_t(y) * w + x) * 4;
+            px[0] = px[1] = px[2] = 0.0f;
+            px[3] = 1.0f;
+            if (!(cs > 0.0f)) continue;
+            Hit<float> hh = trace<float>(s, eye, d, near_d / cs, far_d / cs, HH_TRACE_OPAQUE, true, brute != 0);
+            if (hh.found) output_buffer_value<float>(s, hh, which, px);
+        }
+    return 0;
+}
+
+// The ray-debug variant (path_trace_debug.rgen): num_rays paths through the
+// window pixel (px, py) of a w x h view (shader pixel (px, h - py)), launch id
+// (i, 0), seed `frame`. Writes up to max_segments records of 9 floats
+// (origin xyz, end xyz, colour rgb), ray by ray in depth order; returns the
+// number of segments produced.
+uint32_t orc_ray_debug(void* sp, const hh_camera* cam, const hh_settings* st, uint32_t w, uint32_t h, uint32_t px,
+                       uint32_t py, uint32_t num_rays, uint32_t frame, float* out, uint32_t max_segments) {
+    Scene& s = *(Scene*)sp;
+    if (!s.committed) return 0;
+    hh_launch_consts c;
+    launch_consts(*cam, w, h, s.num_lights, frame, *st, c);
+    ConstsD cd;
+    launch_consts_d(*cam, w, h, cd);
+    CamR<float> cr = cam_from<float>(c, cd);
+    std::vector<float> segs;
+    Tracer<float> tr{s, st->shadow_ray_bias, s.num_lights, st->max_ray_bounces, st->flags & HH_FLAG_LIGHT_POWER, false, {}};
+    tr.debug = true;
+    tr.segments = &segs;
+    for (uint32_t i = 0; i < num_rays; i++) {
+        RNG rng = rng_init(i, 0, frame);
+        const float r = next_float(rng) * 0.5f + 0.5f;
+        const float g = next_float(rng) * 0.5f + 0.5f;
+        const float b = next_float(rng) * 0.5f + 0.5f;
+        tr.debug_color = V3<float>(r, g, b);
+        V3<float> o, d;
+        generate_ray(cr, px, h - py, w, h, rng, o, d);
+        tr.trace_path(o, d, 0.001f, 0u, V3<float>(1, 1, 1), 0, rng);
+    }
+    const uint32_t n = (uint32_t)(segs.size() / 9);
+    std::memcpy(out, segs.data(), sizeof(float) * 9 * std::min(n, max_segments));
+    return n;
+}
+
 int orc_trace_rays(void* sp, const hh_ray* rays, uint32_t n, uint32_t flags, int brute, int fp64, hh_hit* out) {
     Scene& s = *(Scene*)sp;
     if (!s.committed) return 1;
