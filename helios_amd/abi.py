@@ -90,6 +90,33 @@ assert C.sizeof(LaunchConsts) == 192
 assert C.sizeof(Hit) == 24
 assert C.sizeof(Ray) == 32
 
+# glTF ingestion (hh_scene_sink / hh_gltf_info)
+SINK_ADD_TEXTURE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                               C.POINTER(C.c_uint32))
+SINK_ADD_MATERIAL = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(MaterialDesc), C.POINTER(C.c_uint32))
+SINK_ADD_MESH = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                            C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32))
+SINK_ADD_INSTANCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(InstanceDesc), C.POINTER(C.c_uint32))
+SINK_ADD_LIGHT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(LightDesc), C.POINTER(C.c_uint32))
+
+
+class SceneSink(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("add_texture", SINK_ADD_TEXTURE), ("add_material", SINK_ADD_MATERIAL),
+                ("add_mesh", SINK_ADD_MESH), ("add_instance", SINK_ADD_INSTANCE), ("add_light", SINK_ADD_LIGHT)]
+
+
+class GltfInfo(C.Structure):
+    _fields_ = [("num_meshes", C.c_uint32), ("num_instances", C.c_uint32), ("num_materials", C.c_uint32),
+                ("num_textures", C.c_uint32), ("num_lights", C.c_uint32), ("num_cameras", C.c_uint32),
+                ("has_camera", C.c_uint32), ("warnings", C.c_uint32), ("num_triangles", C.c_uint64),
+                ("camera", Camera)]
+
+
+GLTF_WARN_TEXCOORD1 = 0x1
+GLTF_WARN_NON_TRIANGLES = 0x2
+SUBMESH_DTYPE = np.dtype([("mat_idx", "<u4"), ("index_count", "<u4"), ("vertex_count", "<u4"),
+                          ("base_vertex", "<u4"), ("base_index", "<u4")])
+
 # numpy views of the array-of-struct records
 VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("tex_coord", "<f4", 4), ("normal", "<f4", 4),
                          ("tangent", "<f4", 4), ("bitangent", "<f4", 4)])
@@ -125,7 +152,7 @@ LIGHT_SPOT = 1
 LIGHT_POINT = 2
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "LIMIT_EXCEEDED", 4: "DEVICE_ERROR",
-          5: "NAN_DETECTED", 6: "NOT_READY"}
+          5: "NAN_DETECTED", 6: "NOT_READY", 7: "IO"}
 
 
 def ptr(a, ctype=C.c_void_p):

@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "bvh_build.hip", "integrator.hip", "sky_host.hip", "denoise.hip"]
-HOST_SOURCES = ["image_io.cpp"]    # host-only C++ (g++), linked with zlib
+HOST_SOURCES = ["image_io.cpp", "scene_io.cpp"]    # host-only C++ (g++), linked with zlib
 CXX = os.environ.get("CXX", "g++")
 
 

@@ -9,6 +9,7 @@
 // decodes to the same pixels.
 #include <zlib.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -118,4 +119,151 @@ extern "C" hh_status hh_write_png(const char* path, const uint8_t* rgba8, uint32
     if (wrote != file.size() || closed != 0)
         return hh_internal_fail(HH_ERR_IO, (std::string("write failed: ") + path).c_str());
     return HH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PNG input for scene textures: every colour type (grey, RGB, palette,
+// grey+alpha, RGBA) at the bit depths the specification allows for it
+// (1/2/4/8/16), tRNS transparency (palette alpha and colour keys),
+// non-interlaced. The result is RGBA8: 16-bit samples keep their high byte,
+// sub-byte grey is scaled to 0..255.
+// ---------------------------------------------------------------------------
+namespace {
+uint32_t be32(const uint8_t* p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]; }
+}  // namespace
+
+bool hh_internal_png_decode(const uint8_t* data, size_t n, std::vector<uint8_t>& rgba, uint32_t& width,
+                            uint32_t& height, std::string& err) {
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    if (n < 8 || std::memcmp(data, sig, 8) != 0) { err = "not a PNG file"; return false; }
+    uint32_t w = 0, h = 0, depth = 0, ctype = 0, interlace = 0;
+    bool have_ihdr = false, have_end = false;
+    std::vector<uint8_t> idat, plte, trns;
+    size_t pos = 8;
+    while (pos + 12 <= n) {
+        const uint32_t len = be32(data + pos);
+        if (len > n - pos - 12) { err = "truncated chunk"; return false; }
+        const uint8_t* type = data + pos + 4;
+        const uint8_t* body = data + pos + 8;
+        uLong crc = crc32(0L, Z_NULL, 0);
+        crc = crc32(crc, type, uInt(4 + len));
+        if (uint32_t(crc) != be32(body + len)) { err = "chunk CRC mismatch"; return false; }
+        if (!std::memcmp(type, "IHDR", 4)) {
+            if (len != 13) { err = "bad IHDR"; return false; }
+            w = be32(body);
+            h = be32(body + 4);
+            depth = body[8];
+            ctype = body[9];
+            interlace = body[12];
+            if (body[10] != 0 || body[11] != 0) { err = "unknown compression or filter method"; return false; }
+            have_ihdr = true;
+        } else if (!std::memcmp(type, "PLTE", 4)) {
+            plte.assign(body, body + len);
+        } else if (!std::memcmp(type, "tRNS", 4)) {
+            trns.assign(body, body + len);
+        } else if (!std::memcmp(type, "IDAT", 4)) {
+            idat.insert(idat.end(), body, body + len);
+        } else if (!std::memcmp(type, "IEND", 4)) {
+            have_end = true;
+            break;
+        }
+        pos += 12 + size_t(len);
+    }
+    if (!have_ihdr || !have_end) { err = "missing IHDR or IEND"; return false; }
+    if (w == 0 || h == 0 || w > (1u << 15) || h > (1u << 15)) { err = "unsupported PNG size"; return false; }
+    if (interlace != 0) { err = "interlaced PNG is not supported"; return false; }
+    int channels = 0;
+    switch (ctype) {
+        case 0: channels = 1; break;
+        case 2: channels = 3; break;
+        case 3: channels = 1; break;
+        case 4: channels = 2; break;
+        case 6: channels = 4; break;
+        default: err = "bad colour type"; return false;
+    }
+    const bool depth_ok = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                          (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                          ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+    if (!depth_ok) { err = "bad bit depth for the colour type"; return false; }
+    if (ctype == 3 && (plte.empty() || plte.size() % 3 != 0)) { err = "palette image without PLTE"; return false; }
+    const size_t bits = size_t(channels) * depth;
+    const size_t stride = (size_t(w) * bits + 7) / 8;
+    const size_t bpp = std::max<size_t>(1, bits / 8);
+    std::vector<uint8_t> raw((stride + 1) * h);
+    uLongf raw_len = uLongf(raw.size());
+    if (uncompress(raw.data(), &raw_len, idat.data(), uLong(idat.size())) != Z_OK || raw_len != raw.size()) {
+        err = "corrupt image data";
+        return false;
+    }
+    // undo the per-row filters in place
+    std::vector<uint8_t> zero(stride, 0);
+    for (uint32_t r = 0; r < h; r++) {
+        uint8_t* line = raw.data() + size_t(r) * (stride + 1);
+        const uint8_t f = line[0];
+        uint8_t* cur = line + 1;
+        const uint8_t* prev = r ? raw.data() + size_t(r - 1) * (stride + 1) + 1 : zero.data();
+        for (size_t i = 0; i < stride; i++) {
+            const int a = i >= bpp ? cur[i - bpp] : 0;
+            const int b = prev[i];
+            const int c = i >= bpp ? prev[i - bpp] : 0;
+            int pred = 0;
+            switch (f) {
+                case 0: break;
+                case 1: pred = a; break;
+                case 2: pred = b; break;
+                case 3: pred = (a + b) >> 1; break;
+                case 4: pred = paeth(a, b, c); break;
+                default: err = "bad filter type"; return false;
+            }
+            cur[i] = uint8_t(cur[i] + pred);
+        }
+    }
+    rgba.assign(size_t(w) * h * 4, 255);
+    const int maxv = (1 << std::min<uint32_t>(depth, 8)) - 1;
+    for (uint32_t r = 0; r < h; r++) {
+        const uint8_t* s = raw.data() + size_t(r) * (stride + 1) + 1;
+        uint8_t* o = rgba.data() + size_t(r) * w * 4;
+        for (uint32_t x = 0; x < w; x++) {
+            // raw samples (full 16-bit values kept for colour-key comparison)
+            uint32_t v[4] = {0, 0, 0, 0};
+            for (int c = 0; c < channels; c++) {
+                if (depth == 16) {
+                    const uint8_t* p = s + (size_t(x) * channels + c) * 2;
+                    v[c] = (uint32_t(p[0]) << 8) | p[1];
+                } else if (depth == 8) {
+                    v[c] = s[size_t(x) * channels + c];
+                } else {
+                    const size_t bit = size_t(x) * depth;
+                    v[c] = (s[bit / 8] >> (8 - depth - bit % 8)) & uint32_t(maxv);
+                }
+            }
+            auto to8 = [&](uint32_t x8) -> uint8_t {
+                if (depth == 16) return uint8_t(x8 >> 8);
+                if (depth == 8) return uint8_t(x8);
+                return uint8_t((x8 * 255) / uint32_t(maxv));
+            };
+            uint8_t* px = o + size_t(x) * 4;
+            if (ctype == 3) {
+                const uint32_t k = v[0];
+                if (3 * k + 2 >= plte.size()) { err = "palette index out of range"; return false; }
+                px[0] = plte[3 * k];
+                px[1] = plte[3 * k + 1];
+                px[2] = plte[3 * k + 2];
+                px[3] = k < trns.size() ? trns[k] : 255;
+            } else if (ctype == 0 || ctype == 4) {
+                px[0] = px[1] = px[2] = to8(v[0]);
+                px[3] = ctype == 4 ? to8(v[1]) : 255;
+                if (ctype == 0 && trns.size() >= 2 && v[0] == ((uint32_t(trns[0]) << 8) | trns[1])) px[3] = 0;
+            } else {
+                for (int c = 0; c < 3; c++) px[c] = to8(v[c]);
+                px[3] = ctype == 6 ? to8(v[3]) : 255;
+                if (ctype == 2 && trns.size() >= 6 && v[0] == ((uint32_t(trns[0]) << 8) | trns[1]) &&
+                    v[1] == ((uint32_t(trns[2]) << 8) | trns[3]) && v[2] == ((uint32_t(trns[4]) << 8) | trns[5]))
+                    px[3] = 0;
+            }
+        }
+    }
+    width = w;
+    height = h;
+    return true;
 }

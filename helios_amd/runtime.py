@@ -17,7 +17,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import abi
+from . import abi, scenegen
 from .abi import ptr
 
 LIB_PATH = Path(__file__).resolve().parent / "libhelios_hip.so"
@@ -88,6 +88,8 @@ _SIGS = {
     "hh_context_read_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "hh_context_reset_profile": [C.c_void_p],
     "hh_bandwidth_probe": [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)],
+    "hh_scene_load_gltf": [C.c_void_p, C.c_char_p, C.POINTER(abi.GltfInfo)],
+    "hh_gltf_load": [C.c_char_p, C.POINTER(abi.SceneSink), C.POINTER(abi.GltfInfo)],
     "hh_render_output_buffer": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_void_p],
     "hh_surface_save_png": [C.c_void_p, C.c_float, C.c_uint32, C.c_char_p],
     "hh_write_png": [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32],
@@ -238,6 +240,13 @@ class Scene:
 
     def commit(self):
         check(lib().hh_scene_commit(self.h))
+
+    def load_gltf(self, path):
+        """Add the contents of a glTF 2.0 file (.gltf / .glb); returns gltf_info
+        as a dict (counts, warnings, 'camera' or None). Commit afterwards."""
+        info = abi.GltfInfo()
+        check(lib().hh_scene_load_gltf(self.h, str(path).encode(), C.byref(info)))
+        return _gltf_info(info)
 
     def num_lights(self):
         n = C.c_uint32()
@@ -437,6 +446,111 @@ class PathIntegrator:
         if self.h:
             lib().hh_integrator_destroy(self.h)
             self.h = None
+
+
+def _gltf_info(info):
+    d = {k: int(getattr(info, k)) for k, _ in abi.GltfInfo._fields_ if k != "camera"}
+    cam = info.camera
+    d["camera"] = dict(position=tuple(cam.position), orientation=tuple(cam.orientation), fov_y_deg=cam.fov_y_deg,
+                       near_plane=cam.near_plane, far_plane=cam.far_plane, focal_length=cam.focal_length,
+                       aperture_radius=cam.aperture_radius) if info.has_camera else None
+    return d
+
+
+def gltf_load(path, sink):
+    """hh_gltf_load with a Python sink: an object with add_texture(w, h, fmt,
+    texels (h, w, 4)), add_material(dict), add_mesh(vertices, indices,
+    submeshes, materials), add_instance(dict), add_light(dict), each returning
+    the new id. Returns gltf_info as a dict."""
+    errors = []
+
+    def guard(fn):
+        def cb(*args):
+            try:
+                return fn(*args)
+            except Exception as e:  # reported after the C call returns
+                errors.append(e)
+                return 1
+        return cb
+
+    def fields(struct):
+        return {k: (list(getattr(struct, k)) if isinstance(getattr(struct, k), C.Array) else getattr(struct, k))
+                for k, _ in struct._fields_}
+
+    def array(addr, ctype, n):
+        return np.ctypeslib.as_array((ctype * n).from_address(addr)).copy()
+
+    def add_texture(user, w, h, fmt, texels, out):
+        ct = C.c_float if fmt == abi.TEXTURE_RGBA32F else C.c_uint8
+        out[0] = sink.add_texture(w, h, fmt, array(texels, ct, w * h * 4).reshape(h, w, 4))
+        return 0
+
+    def add_material(user, d, out):
+        out[0] = sink.add_material(fields(d.contents))
+        return 0
+
+    def add_mesh(user, v, nv, ix, ni, sm, ns, mats, nm, out):
+        verts = array(v, C.c_uint8, nv * abi.VERTEX_DTYPE.itemsize).view(abi.VERTEX_DTYPE)
+        subs = array(sm, C.c_uint8, ns * abi.SUBMESH_DTYPE.itemsize).view(abi.SUBMESH_DTYPE)
+        out[0] = sink.add_mesh(verts, array(ix, C.c_uint32, ni), subs, array(mats, C.c_uint32, nm))
+        return 0
+
+    def add_instance(user, d, out):
+        out[0] = sink.add_instance(fields(d.contents))
+        return 0
+
+    def add_light(user, d, out):
+        out[0] = sink.add_light(fields(d.contents))
+        return 0
+
+    cbs = (abi.SINK_ADD_TEXTURE(guard(add_texture)), abi.SINK_ADD_MATERIAL(guard(add_material)),
+           abi.SINK_ADD_MESH(guard(add_mesh)), abi.SINK_ADD_INSTANCE(guard(add_instance)),
+           abi.SINK_ADD_LIGHT(guard(add_light)))
+    s = abi.SceneSink(None, *cbs)
+    info = abi.GltfInfo()
+    st = lib().hh_gltf_load(str(path).encode(), C.byref(s), C.byref(info))
+    if errors:
+        raise errors[0]
+    check(st)
+    return _gltf_info(info)
+
+
+class _DescSink:
+    def __init__(self, desc):
+        self.d = desc
+
+    def add_texture(self, w, h, fmt, texels):
+        self.d.textures.append((w, h, fmt, texels))
+        return len(self.d.textures) - 1
+
+    def add_material(self, m):
+        self.d.materials.append(m)
+        return len(self.d.materials) - 1
+
+    def add_mesh(self, verts, indices, subs, mats):
+        sm = [tuple(int(x) for x in r) for r in subs[["mat_idx", "index_count", "vertex_count", "base_vertex",
+                                                      "base_index"]].tolist()]
+        self.d.meshes.append(scenegen.MeshDesc(verts, indices, sm, [int(m) for m in mats]))
+        return len(self.d.meshes) - 1
+
+    def add_instance(self, inst):
+        self.d.instances.append((np.asarray(inst["model"], np.float32), np.asarray(inst["normal"], np.float32),
+                                 inst["mesh"], inst["material_override"]))
+        return len(self.d.instances) - 1
+
+    def add_light(self, light):
+        self.d.lights.append(light)
+        return len(self.d.lights) - 1
+
+
+def gltf_scene_desc(path, width=256, height=256, max_ray_bounces=5, spp=1):
+    """A scenegen.SceneDesc holding what the glTF loader produces for `path`
+    (camera = the file's first camera when it has one); returns (desc, info)."""
+    d = scenegen.SceneDesc(width=width, height=height, max_ray_bounces=max_ray_bounces, spp=spp)
+    info = gltf_load(path, _DescSink(d))
+    if info["camera"] is not None:
+        d.camera = dict(info["camera"])
+    return d, info
 
 
 def render_output_buffer(scene, camera, which, surface):

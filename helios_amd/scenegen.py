@@ -534,3 +534,183 @@ def combined_workload(seed=0, width=3840, height=2160, n_meshes=40, n_instances=
                                    near=1.0, far=2000.0, aperture=0.0))
     d.camera = cams[0]
     return d, cams
+
+
+# ---------------------------------------------------------------------------
+# glTF 2.0 export of a SceneDesc (the synthetic workloads as files)
+# ---------------------------------------------------------------------------
+def _png_bytes(rgba8):
+    """Minimal PNG (8-bit RGBA, filter 0 on every row) of an (H, W, 4) array."""
+    import struct
+    import zlib
+    a = np.ascontiguousarray(rgba8, np.uint8)
+    h, w, _ = a.shape
+    raw = b"".join(b"\x00" + a[r].tobytes() for r in range(h))
+
+    def chunk(t, body):
+        return struct.pack(">I", len(body)) + t + body + struct.pack(">I", zlib.crc32(t + body) & 0xFFFFFFFF)
+
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+
+
+def _quat_looking_along(fwd):
+    """(x, y, z, w) of a rotation taking -Z to fwd (glTF lights point along -Z)."""
+    f = np.asarray(fwd, np.float64)
+    f = f / np.linalg.norm(f)
+    z = -f
+    a = np.array([0.0, 1.0, 0.0]) if abs(z[1]) < 0.99 else np.array([1.0, 0.0, 0.0])
+    x = np.cross(a, z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    return quat_from_matrix(np.stack([x, y, z], axis=1))
+
+
+def write_gltf(desc, path):
+    """Write `desc` as glTF 2.0: `path` ending in .glb -> one binary file, else
+    .gltf + a .bin beside it. Meshes -> meshes (a primitive per submesh, with
+    TANGENT w = handedness of the stored bitangent), instances -> nodes with
+    their model matrix, materials -> metallic-roughness materials (linear
+    baseColorFactor = the packed albedo pow(v, 2.2); emission above 1 through
+    KHR_materials_emissive_strength; alpha-tested -> MASK at cutoff 0.1,
+    transparent -> BLEND), sRGB RGBA8 albedo textures -> embedded PNGs, the
+    camera -> a camera node, lights -> KHR_lights_punctual (their radius is
+    not representable). Material overrides, IBL cube maps, normal / roughness
+    / metallic / emissive textures and the sky are not exported."""
+    import json
+    import struct
+    from pathlib import Path
+    path = Path(path)
+    blob = bytearray()
+
+    def add_view(data, target=None):
+        while len(blob) % 4:
+            blob.append(0)
+        off = len(blob)
+        blob.extend(data)
+        v = {"buffer": 0, "byteOffset": off, "byteLength": len(data)}
+        if target:
+            v["target"] = target
+        views.append(v)
+        return len(views) - 1
+
+    def add_accessor(arr, gltf_type, ctype, target=None, minmax=False):
+        arr = np.ascontiguousarray(arr)
+        acc = {"bufferView": add_view(arr.tobytes(), target), "componentType": ctype, "count": int(len(arr)),
+               "type": gltf_type}
+        if minmax:
+            acc["min"] = [float(x) for x in arr.min(axis=0)]
+            acc["max"] = [float(x) for x in arr.max(axis=0)]
+        accessors.append(acc)
+        return len(accessors) - 1
+
+    views, accessors, images, textures, materials, meshes, nodes = [], [], [], [], [], [], []
+    tex_index = {}
+    for ti, (w, h, fmt, data) in enumerate(desc.textures):
+        if fmt != TEXTURE_RGBA8_SRGB:
+            continue
+        images.append({"bufferView": add_view(_png_bytes(np.asarray(data, np.uint8).reshape(h, w, 4))),
+                       "mimeType": "image/png"})
+        textures.append({"source": len(images) - 1})
+        tex_index[ti] = len(textures) - 1
+    for m in desc.materials:
+        for k in ("normal_texture", "roughness_texture", "metallic_texture", "emissive_texture"):
+            if m.get(k, -1) != -1:
+                raise NotImplementedError(f"export of {k}")
+        pbr = {"metallicFactor": float(m["metallic_value"]), "roughnessFactor": float(m["roughness_value"])}
+        alb = m["albedo_value"]
+        if m.get("albedo_texture", -1) != -1:
+            if m["albedo_texture"] not in tex_index:
+                raise NotImplementedError("albedo textures other than sRGB RGBA8")
+            pbr["baseColorTexture"] = {"index": tex_index[m["albedo_texture"]]}
+        else:
+            pbr["baseColorFactor"] = [float(np.float32(alb[k]) ** np.float32(2.2)) for k in range(3)] + [float(alb[3])]
+        mat = {"pbrMetallicRoughness": pbr}
+        em = [float(x) for x in m["emissive_value"][:3]]
+        if max(em) > 0.0:
+            peak = max(em)
+            if peak > 1.0:
+                mat["emissiveFactor"] = [x / peak for x in em]
+                mat["extensions"] = {"KHR_materials_emissive_strength": {"emissiveStrength": peak}}
+            else:
+                mat["emissiveFactor"] = em
+        if m.get("type", 0) == 1:
+            mat["alphaMode"] = "BLEND"
+        elif m.get("alpha_test", 0):
+            mat["alphaMode"] = "MASK"
+            mat["alphaCutoff"] = 0.1
+        materials.append(mat)
+    for me in desc.meshes:
+        prims = []
+        for (mat_idx, index_count, vertex_count, base_vertex, base_index) in me.submeshes:
+            v = me.vertices[base_vertex:base_vertex + vertex_count]
+            idx = me.indices[base_index:base_index + index_count].astype(np.int64) - base_vertex
+            if idx.min() < 0 or idx.max() >= vertex_count:
+                raise NotImplementedError("submesh indices outside its vertex range")
+            n = v["normal"][:, :3].astype(np.float32)
+            t = v["tangent"][:, :3].astype(np.float32)
+            b = v["bitangent"][:, :3].astype(np.float64)
+            hand = np.where(np.sum(np.cross(n.astype(np.float64), t.astype(np.float64)) * b, axis=1) < 0, -1.0, 1.0)
+            attrs = {"POSITION": add_accessor(v["position"][:, :3].astype(np.float32), "VEC3", 5126, 34962, True),
+                     "NORMAL": add_accessor(n, "VEC3", 5126, 34962),
+                     "TEXCOORD_0": add_accessor(v["tex_coord"][:, :2].astype(np.float32), "VEC2", 5126, 34962),
+                     "TANGENT": add_accessor(np.concatenate([t, hand[:, None].astype(np.float32)], axis=1), "VEC4",
+                                             5126, 34962)}
+            prims.append({"attributes": attrs, "indices": add_accessor(idx.astype(np.uint32), "SCALAR", 5125, 34963),
+                          "material": int(me.materials[mat_idx])})
+        meshes.append({"primitives": prims})
+    for (model, normal, mesh, override) in desc.instances:
+        if override != -1:
+            raise NotImplementedError("material overrides")
+        nodes.append({"mesh": int(mesh), "matrix": [float(x) for x in np.asarray(model, np.float32).ravel()]})
+    cameras = []
+    if desc.camera:
+        c = desc.camera
+        cameras.append({"type": "perspective", "perspective": {
+            "yfov": math.radians(c["fov_y_deg"]), "znear": float(c["near_plane"]), "zfar": float(c["far_plane"])}})
+        nodes.append({"camera": 0, "translation": [float(x) for x in c["position"]],
+                      "rotation": [float(x) for x in c["orientation"]]})
+    lights = []
+    kinds = {LIGHT_DIRECTIONAL: "directional", LIGHT_POINT: "point", LIGHT_SPOT: "spot"}
+    for l in desc.lights:
+        gl = {"type": kinds[l["type"]], "color": [float(x) for x in l["color"]], "intensity": float(l["intensity"])}
+        if l["type"] == LIGHT_SPOT:
+            gl["spot"] = {"innerConeAngle": math.radians(l["inner_cone_deg"]),
+                          "outerConeAngle": math.radians(l["outer_cone_deg"])}
+        lights.append(gl)
+        node = {"extensions": {"KHR_lights_punctual": {"light": len(lights) - 1}}}
+        if l["type"] != LIGHT_DIRECTIONAL:
+            node["translation"] = [float(x) for x in l["position"]]
+        if l["type"] != LIGHT_POINT:
+            node["rotation"] = [float(x) for x in _quat_looking_along(l["forward"])]
+        nodes.append(node)
+    doc = {"asset": {"version": "2.0", "generator": "helios_amd.scenegen"}, "scene": 0,
+           "scenes": [{"nodes": list(range(len(nodes)))}], "nodes": nodes, "meshes": meshes,
+           "materials": materials, "accessors": accessors, "bufferViews": views}
+    if images:
+        doc["images"], doc["textures"] = images, textures
+    if cameras:
+        doc["cameras"] = cameras
+    used = []
+    if lights:
+        doc["extensions"] = {"KHR_lights_punctual": {"lights": lights}}
+        used.append("KHR_lights_punctual")
+    if any("extensions" in m for m in materials):
+        used.append("KHR_materials_emissive_strength")
+    if used:
+        doc["extensionsUsed"] = used
+    while len(blob) % 4:
+        blob.append(0)
+    if path.suffix.lower() == ".glb":
+        doc["buffers"] = [{"byteLength": len(blob)}]
+        js = json.dumps(doc).encode()
+        js += b" " * (-len(js) % 4)
+        out = (struct.pack("<III", 0x46546C67, 2, 12 + 8 + len(js) + 8 + len(blob)) +
+               struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(blob), 0x004E4942) + bytes(blob))
+        path.write_bytes(out)
+    else:
+        binp = path.with_suffix(".bin")
+        binp.write_bytes(bytes(blob))
+        doc["buffers"] = [{"byteLength": len(blob), "uri": binp.name}]
+        path.write_text(json.dumps(doc))
+    return path

@@ -282,6 +282,46 @@ hh_status hh_sky_fit(const hh_sky_params* params, const float sun_dir[3], float 
 hh_status hh_scene_bvh_stats(hh_scene scene, uint64_t out[4]);
 
 /* ---------------------------------------------------------------------------
+ * Scene files: glTF 2.0 (.gltf with external or data-URI buffers, or .glb),
+ * the stand-in for ResourceManager's AssetCore loading
+ * (src/engine/core/resource_manager.cpp:48-107,196-222,296-490,494-672),
+ * whose formats are not part of the reference. The loader hands every
+ * resource to a sink (ResourceManager's role); hh_scene_load_gltf binds the
+ * sink to a scene, which the caller then commits. Mapping: mesh -> mesh
+ * (first use), primitive -> submesh with its material, mesh node -> instance
+ * (normal matrix = global transform without scale), baseColorFactor ->
+ * albedo_value^(2.2) = factor, metallicRoughnessTexture -> roughness channel
+ * 1 / metallic channel 2, factors other than 1 multiplied into a float copy
+ * of the texture, alphaMode MASK -> alpha thresholded at alphaCutoff and
+ * alpha-tested, BLEND -> transparent, emissiveFactor x
+ * KHR_materials_emissive_strength -> emissive, first perspective camera ->
+ * info.camera (focal length 8, aperture 0.1: CameraNode's defaults),
+ * KHR_lights_punctual -> lights of radius 0. PNG images only. On failure the
+ * sink may already hold part of the file. */
+typedef struct hh_scene_sink {
+    void* user;
+    hh_status (*add_texture)(void* user, uint32_t width, uint32_t height, uint32_t format, const void* texels,
+                             uint32_t* out_id);
+    hh_status (*add_material)(void* user, const hh_material_desc* desc, uint32_t* out_id);
+    hh_status (*add_mesh)(void* user, const hh_vertex* vertices, uint32_t num_vertices, const uint32_t* indices,
+                          uint32_t num_indices, const hh_submesh* submeshes, uint32_t num_submeshes,
+                          const uint32_t* material_ids, uint32_t num_materials, uint32_t* out_id);
+    hh_status (*add_instance)(void* user, const hh_instance_desc* desc, uint32_t* out_id);
+    hh_status (*add_light)(void* user, const hh_light_desc* desc, uint32_t* out_id);
+} hh_scene_sink;
+#define HH_GLTF_WARN_TEXCOORD1 0x1u       /* a texture asked for TEXCOORD_1; TEXCOORD_0 was used */
+#define HH_GLTF_WARN_NON_TRIANGLES 0x2u   /* point / line primitives were skipped */
+typedef struct hh_gltf_info {
+    uint32_t num_meshes, num_instances, num_materials, num_textures, num_lights, num_cameras;
+    uint32_t has_camera;      /* camera below is valid */
+    uint32_t warnings;        /* HH_GLTF_WARN_* */
+    uint64_t num_triangles;
+    hh_camera camera;         /* the first perspective camera in node order */
+} hh_gltf_info;
+hh_status hh_gltf_load(const char* path, const hh_scene_sink* sink, hh_gltf_info* info);
+hh_status hh_scene_load_gltf(hh_scene scene, const char* path, hh_gltf_info* info);
+
+/* ---------------------------------------------------------------------------
  * Output surface (replaces the RGBA32F ping-pong images)
  * ------------------------------------------------------------------------- */
 hh_status hh_surface_create(hh_context ctx, uint32_t width, uint32_t height, hh_surface* out);
