@@ -811,20 +811,31 @@ hh_status hh_scene_commit(hh_scene s) {
     for (uint32_t t : mesh_tris) node_cap += t + 1;
     void* d_nodes = nullptr;
     float4* d_tris = nullptr;
+    float4* d_shade = nullptr;
     HH_HIP(dalloc(s, node_cap * node_bytes, &d_nodes));
     HH_HIP(dalloc(s, ntri * 3 * sizeof(float4), (void**)&d_tris));
+    HH_HIP(dalloc(s, ntri * 4 * sizeof(float4), (void**)&d_shade));
     std::vector<uint32_t> blas_root(global_meshes.size());
     std::vector<float> blas_box(global_meshes.size() * 6);
     uint32_t node_base = 0, tri_base = 0;
     float4* d_soup = nullptr;
+    float4* d_shade_soup = nullptr;
     size_t soup_cap = 0;
     for (uint32_t mt : mesh_tris) soup_cap = std::max<size_t>(soup_cap, mt);
     HH_HIP(hipMalloc(&d_soup, soup_cap * 3 * sizeof(float4)));
-    std::vector<float4> soup;
+    if (hipMalloc(&d_shade_soup, soup_cap * 4 * sizeof(float4)) != hipSuccess) {
+        (void)hipFree(d_soup);
+        return fail(HH_ERR_OUT_OF_MEMORY, "shading records");
+    }
+    // Per-triangle shading record (4 float4, leaf order like the positions):
+    // (u0 v0 u1 v1) (u2 v2 n0.x n0.y) (n0.z n1.xyz) (n2.xyz 0): the vertex
+    // attributes the shading and the alpha test interpolate, in one line.
+    std::vector<float4> soup, shade_soup;
     hh_status st = HH_OK;
     for (size_t g = 0; g < global_meshes.size() && st == HH_OK; g++) {
         const MeshHost& me = s->meshes[global_meshes[g]];
         soup.clear();
+        shade_soup.clear();
         for (uint32_t gi = 0; gi < me.sm.size(); gi++) {
             const hh_submesh& sm = me.sm[gi];
             const hh_material_desc& m = s->materials[me.mats[sm.mat_idx]];
@@ -839,15 +850,25 @@ hh_status hh_scene_commit(hh_scene s) {
                 soup.push_back(make_float4(v3[0]->position[0], v3[0]->position[1], v3[0]->position[2], fw0));
                 soup.push_back(make_float4(v3[1]->position[0], v3[1]->position[1], v3[1]->position[2], fw1));
                 soup.push_back(make_float4(v3[2]->position[0], v3[2]->position[1], v3[2]->position[2], 0.f));
+                shade_soup.push_back(make_float4(v3[0]->tex_coord[0], v3[0]->tex_coord[1], v3[1]->tex_coord[0], v3[1]->tex_coord[1]));
+                shade_soup.push_back(make_float4(v3[2]->tex_coord[0], v3[2]->tex_coord[1], v3[0]->normal[0], v3[0]->normal[1]));
+                shade_soup.push_back(make_float4(v3[0]->normal[2], v3[1]->normal[0], v3[1]->normal[1], v3[1]->normal[2]));
+                shade_soup.push_back(make_float4(v3[2]->normal[0], v3[2]->normal[1], v3[2]->normal[2], 0.f));
             }
         }
-        if (soup.empty())
+        if (soup.empty()) {
             for (int k = 0; k < 3; k++) soup.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+            for (int k = 0; k < 4; k++) shade_soup.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+        }
         uint32_t n = (uint32_t)(soup.size() / 3);
         hipError_t e = hipMemcpyAsync(d_soup, soup.data(), soup.size() * sizeof(float4), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_shade_soup, shade_soup.data(), shade_soup.size() * sizeof(float4), hipMemcpyHostToDevice,
+                               ctx->stream);
         uint32_t nn = 0;
         if (e == hipSuccess)
-            e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup, d_tris, tri_base, d_nodes, node_base, ctx->bs, &nn,
+            e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup, d_tris, d_shade_soup, d_shade, tri_base, d_nodes,
+                           node_base, ctx->bs, &nn,
                            &blas_box[g * 6]);
         if (e != hipSuccess) {
             st = hip_fail(e, "BLAS build");
@@ -859,6 +880,7 @@ hh_status hh_scene_commit(hh_scene s) {
     }
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_soup);
+    (void)hipFree(d_shade_soup);
     if (st != HH_OK) return st;
     s->bvh_stats[0] = node_base;
     s->bvh_stats[1] = tri_base;
@@ -1018,6 +1040,7 @@ hh_status hh_scene_commit(hh_scene s) {
     SceneGPU g{};
     g.nodes = d_nodes;
     g.tris = d_tris;
+    g.shade = d_shade;
     g.vtx = d_vtx;
     g.idx = d_idx;
     g.env = d_env;

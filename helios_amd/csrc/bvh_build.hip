@@ -286,6 +286,16 @@ __global__ void k_gather_tris(const float4* __restrict__ src, const uint32_t* __
     dst[3 * i + 2] = src[3 * p + 2];
 }
 
+// The per-triangle shading records follow their triangles into leaf order.
+__global__ void k_gather_shade(const float4* __restrict__ src, const uint32_t* __restrict__ order, uint32_t n,
+                               float4* __restrict__ dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = order[i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[4 * i + k] = src[4 * p + k];
+}
+
 // Quantization of one axis of one child box: the decoded planes
 // fmaf(q, 2^e, origin) must satisfy lo' <= lo and hi' >= hi exactly in float.
 __device__ inline bool quant_axis(float o, float s, float clo, float chi, uint8_t& qlo, uint8_t& qhi) {
@@ -449,7 +459,7 @@ void BuildScratch::release() {
 // writes the leaf-ordered triangles to d_tri_pool + tri_base and the nodes to
 // d_node_pool + node_base (global indices in child references).
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
-                      uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
+                      const float4* d_shade_src, float4* d_shade_pool, uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
                       float root_box[6]) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
@@ -465,6 +475,7 @@ hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, cons
                           tri_base, leaf_size, 0, order, sc, num_nodes, root_box);
     HH_TRY(e);
     k_gather_tris<<<blocks(n, B), B, 0, st>>>(d_tri_src, order, n, d_tri_pool + 3 * (size_t)tri_base);
+    k_gather_shade<<<blocks(n, B), B, 0, st>>>(d_shade_src, order, n, d_shade_pool + 4 * (size_t)tri_base);
     return hipGetLastError();
 }
 

@@ -69,7 +69,9 @@ __global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0
     constexpr int TR = 4 + 2, TC = TW + 2;   // halo tile rows / columns
     constexpr int NG = (CT + 31) / 32;       // channel groups of 32
     static_assert(C0 % 8 == 0 && C1 % 8 == 0 && COUT % 16 == 0, "channel groups");
-    __shared__ bf16x8 tile[TR * TC * 4];     // [row][col][4 chunks of 8 channels]
+    constexpr int TILE = TR * TC * 4;
+    constexpr int SPT = (TILE + 255) / 256;  // staged 16-B chunks per thread
+    __shared__ bf16x8 tile[TILE];            // [row][col][4 chunks of 8 channels]
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t r = lane & 15u, g = lane >> 4;
     const uint32_t segs = (W + TW - 1) / TW, rblocks = (H + 3) / 4;
@@ -86,16 +88,20 @@ __global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int cg = 0; cg < NG; cg++) {
-        __syncthreads();
-        // stage the halo tile of channels [32 cg, 32 cg + 32)
-        for (int e = (int)threadIdx.x; e < TR * TC * 4; e += 256) {
+    // Halo tile of channels [32 cg, 32 cg + 32): every load of a thread is
+    // issued before any is stored, and before the barrier that waits for the
+    // previous group's MFMAs (the staging registers are dead during them).
+    bf16x8 stage[SPT];
+    auto load_group = [&](int cg) {
+#pragma unroll
+        for (int i = 0; i < SPT; i++) {
+            const int e = (int)threadIdx.x + i * 256;
             const int chunk = e & 3, pix = e >> 2;
             const int ty = pix / TC, tx = pix - ty * TC;
             const int sy = y0 - 1 + ty, sx = x0 - 1 + tx;
             const int c = cg * 32 + chunk * 8;
             bf16x8 v = zero8();
-            if (c < CT && sy >= 0 && sy < (int)H && sx >= 0 && sx < (int)W && n < N) {
+            if (e < TILE && c < CT && sy >= 0 && sy < (int)H && sx >= 0 && sx < (int)W && n < N) {
                 const __bf16* p;
                 if (C1 == 0 || c < C0)
                     p = src0 + (((size_t)n * H0 + (uint32_t)sy / UP0) * W0 + (uint32_t)sx / UP0) * C0 + c;
@@ -103,7 +109,16 @@ __global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0
                     p = src1 + (((size_t)n * H + (uint32_t)sy) * W + (uint32_t)sx) * C1 + (c - C0);
                 v = *(const bf16x8*)p;
             }
-            tile[e] = v;
+            stage[i] = v;
+        }
+    };
+    for (int cg = 0; cg < NG; cg++) {
+        load_group(cg);
+        __syncthreads();  // every wave is done reading the previous group
+#pragma unroll
+        for (int i = 0; i < SPT; i++) {
+            const int e = (int)threadIdx.x + i * 256;
+            if (e < TILE) tile[e] = stage[i];
         }
         __syncthreads();
         const int c = cg * 32 + (int)g * 8;      // this lane's 8 channels

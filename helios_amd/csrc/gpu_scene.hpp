@@ -112,6 +112,7 @@ struct TexDesc {
 struct SceneGPU {
     const void* nodes;            // WideNode<W> pool
     const float4* tris;           // 3 per triangle
+    const float4* shade;          // 4 per triangle: uv0 uv1 | uv2 n0.xy | n0.z n1 | n2 (leaf order)
     const InstTrav* inst_trav;
     const InstanceData* inst;
     const MaterialData* mats;

@@ -158,26 +158,25 @@ __device__ inline f3 m3_mul(const float* m, f3 v) {
 }
 
 // populate_surface_properties (path_trace_rchit.glsl:256-280)
+// Positions come from the traversal's triangle record and normals / UVs from
+// the shading record of the same triangle (the vertex values themselves, so
+// the interpolation is the vertex path's, bit for bit); only normal-mapped
+// materials read the vertices, for their tangent frames.
 __device__ inline void populate(const SceneGPU& S, uint32_t inst, uint32_t tri, float u, float v, Surface& p) {
     const float4* tw = S.tris + 3 * (size_t)tri;
-    uint32_t prim = __float_as_uint(tw[0].w);
-    uint32_t geom = __float_as_uint(tw[1].w) & 0x7FFFFFFFu;
+    const float4* sr = S.shade + 4 * (size_t)tri;
+    const float4 a = tw[0], b = tw[1], c = tw[2];
+    const float4 r0 = sr[0], r1 = sr[1], r2 = sr[2], r3 = sr[3];
+    uint32_t prim = __float_as_uint(a.w);
+    uint32_t geom = __float_as_uint(b.w) & 0x7FFFFFFFu;
     const InstanceData& I = S.inst[inst];
-    uint32_t mesh = S.inst_trav[inst].mesh;
     uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
     const MaterialData& m = S.mats[info.y];
-    uint32_t pid = prim + info.x;
-    const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * pid;
-    const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
-    const float4* V0 = vb + 5 * (size_t)ix[0];
-    const float4* V1 = vb + 5 * (size_t)ix[1];
-    const float4* V2 = vb + 5 * (size_t)ix[2];
     float b0 = 1.0f - u - v, b1 = u, b2 = v;
-    f3 pos = ld3(V0[0]) * b0 + ld3(V1[0]) * b1 + ld3(V2[0]) * b2;
-    float4 t0 = V0[1], t1 = V1[1], t2 = V2[1];
-    float uv0 = t0.x * b0 + t1.x * b1 + t2.x * b2;
-    float uv1 = t0.y * b0 + t1.y * b1 + t2.y * b2;
-    f3 n = normalize(ld3(V0[2]) * b0 + ld3(V1[2]) * b1 + ld3(V2[2]) * b2);
+    f3 pos = ld3(a) * b0 + ld3(b) * b1 + ld3(c) * b2;
+    float uv0 = r0.x * b0 + r0.z * b1 + r1.x * b2;
+    float uv1 = r0.y * b0 + r0.w * b1 + r1.y * b2;
+    f3 n = normalize(mk(r1.z, r1.w, r2.x) * b0 + mk(r2.y, r2.z, r2.w) * b1 + ld3(r3) * b2);
     p.pos = m4_point(I.model, pos, 1.0f);
     p.vnormal = m3_mul(I.normal, n);
     float4 alb;
@@ -188,6 +187,12 @@ __device__ inline void populate(const SceneGPU& S, uint32_t inst, uint32_t tri, 
     if (m.tex0[1] == -1) {
         p.normal = p.vnormal;
     } else {
+        const uint32_t mesh = S.inst_trav[inst].mesh;
+        const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * (prim + info.x);
+        const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
+        const float4* V0 = vb + 5 * (size_t)ix[0];
+        const float4* V1 = vb + 5 * (size_t)ix[1];
+        const float4* V2 = vb + 5 * (size_t)ix[2];
         f3 tg = normalize(ld3(V0[3]) * b0 + ld3(V1[3]) * b1 + ld3(V2[3]) * b2);
         f3 bt = normalize(ld3(V0[4]) * b0 + ld3(V1[4]) * b1 + ld3(V2[4]) * b2);
         f3 T = normalize(m3_mul(I.normal, tg)), B = normalize(m3_mul(I.normal, bt)), N = normalize(p.vnormal);

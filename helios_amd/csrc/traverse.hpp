@@ -158,19 +158,16 @@ __device__ inline float4 sample_tex(const SceneGPU& S, int slot, float s, float 
     return r;
 }
 
-// path_trace_rahit.glsl:162-188: alpha of the candidate hit.
-__device__ inline float hit_alpha(const SceneGPU& S, uint32_t inst, uint32_t geom, uint32_t prim, float u, float v) {
+// path_trace_rahit.glsl:162-188: alpha of the candidate hit on triangle
+// `tri` (global index; its UVs come from the shading record).
+__device__ inline float hit_alpha(const SceneGPU& S, uint32_t inst, uint32_t geom, uint32_t tri, float u, float v) {
     uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
     const MaterialData& m = S.mats[info.y];
     if (m.tex0[0] == -1) return m.albedo[3];
-    uint32_t mesh = S.inst_trav[inst].mesh;
-    uint32_t p = prim + info.x;
-    const uint32_t* ix = S.idx + S.ibase[mesh] + 3 * p;
-    const float4* vb = S.vtx + 5 * (size_t)S.vbase[mesh];
-    float4 t0 = vb[5 * (size_t)ix[0] + 1], t1 = vb[5 * (size_t)ix[1] + 1], t2 = vb[5 * (size_t)ix[2] + 1];
+    const float4 r0 = S.shade[4 * (size_t)tri], r1 = S.shade[4 * (size_t)tri + 1];
     float bx = (1.0f - u) - v, by = u, bz = v;
-    float tu = (t0.x * bx + t1.x * by) + t2.x * bz;
-    float tv = (t0.y * bx + t1.y * by) + t2.y * bz;
+    float tu = (r0.x * bx + r0.z * by) + r1.x * bz;
+    float tv = (r0.y * bx + r0.w * by) + r1.y * bz;
     return sample_tex(S, m.tex0[0], tu, tv).w;
 }
 
@@ -363,7 +360,7 @@ struct Tracer {
             const uint32_t geom = gw & 0x7FFFFFFFu;
             if (h.found && t == h.t && !key_less((uint32_t)inst, geom, prim, h.inst, hgeom, hprim)) continue;
             if (alpha_on && (gw & 0x80000000u) == 0u) {
-                if (hit_alpha(*S, (uint32_t)inst, geom, prim, u, v) < 0.1f) continue;
+                if (hit_alpha(*S, (uint32_t)inst, geom, ti, u, v) < 0.1f) continue;
             }
             h.found = true;
             h.t = t;
