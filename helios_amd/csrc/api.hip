@@ -387,7 +387,7 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->bvh_width == 4 || t->bvh_width == 8, "bvh_width must be 4 or 8");
     HH_ARG(t->extend_block == 64 || t->extend_block == 128 || t->extend_block == 256, "extend_block must be 64/128/256");
     HH_ARG(t->persistent_blocks_per_cu >= 1 && t->persistent_blocks_per_cu <= 64, "blocks per CU must be 1..64");
-    HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 64, "frames_in_flight must be 1..64");
+    HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 256, "frames_in_flight must be 1..256");
     HH_ARG(t->refill_lanes >= 1 && t->refill_lanes <= 64, "refill_lanes must be 1..64");
     HH_ARG(t->traversal <= 1, "traversal must be 0 (while-while) or 1 (if-if)");
     ctx->tune.refill_lanes = t->refill_lanes;

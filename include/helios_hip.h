@@ -480,7 +480,8 @@ typedef struct hh_tuning {
      * hh_integrator_render_frames is asked for several frames. Samples are
      * still accumulated in frame order, so the image is bit-identical to
      * rendering the frames one at a time; deep-bounce queues are F times
-     * fuller, which hides the latency tail of incoherent rays. */
+     * fuller, which hides the latency tail of incoherent rays. 1..256; the
+     * path state takes about 200 B per pixel and frame in flight. */
     uint32_t frames_in_flight;
     /* 1 = quantized 64-B BVH4 nodes (8-bit child boxes, outward rounded);
      * applies to bvh_width 4 and takes effect at the next scene commit. */
