@@ -168,8 +168,9 @@ def time_to_relmse(ctx, scene, desc, frame_ms, weights, ref_frames, target=0.01)
         if k_plain is not None and k_dn is not None:
             break
         k = max(k + 1, int(k * 1.1))
-    from helios_amd.denoise import forward_flops
+    from helios_amd.denoise import forward_bytes, forward_flops
     flops = forward_flops(h, w)
+    dn_bytes = forward_bytes(h, w)
     from helios_amd.abi import HH_FLAG_MEGAKERNEL
 
     def wall(k, denoise, mega):
@@ -207,6 +208,11 @@ def time_to_relmse(ctx, scene, desc, frame_ms, weights, ref_frames, target=0.01)
         "denoise_ms": round(dn_ms, 3), "throughput_frame_ms": round(frame_ms, 4),
         "denoiser_forward": {"gflop": round(flops / 1e9, 2), "tflops": round(flops / (dn_ms * 1e-3) / 1e12, 1),
                              "frac_of_bf16_mfma_peak": round(flops / (dn_ms * 1e-3) / 2.5e15, 4),
+                             "alg_bytes": dn_bytes, "flop_per_byte": round(flops / dn_bytes, 1),
+                             "hbm_roofline": {"bound": "hbm (below the 312 FLOP/B ridge of 2.5 PFLOP/s over 8 TB/s)",
+                                              "achieved": round(dn_bytes / (dn_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                                              "unit": "GB/s",
+                                              "frac": round(dn_bytes / (dn_ms * 1e-3) / 8e12, 4)},
                              "timing": "host clock around one synchronized hh_denoise call (median of 5)"},
         "plain_frames": k_plain, "denoised_frames": k_dn,
         "plain_s": round(plain_s, 4) if plain_s else None,

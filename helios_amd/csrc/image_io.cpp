@@ -10,6 +10,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -266,4 +267,171 @@ bool hh_internal_png_decode(const uint8_t* data, size_t n, std::vector<uint8_t>&
     width = w;
     height = h;
     return true;
+}
+
+// ---------------------------------------------------------------------------
+// Environment maps for the IBL node (IBLNode, scene.h; the reference loads
+// cube maps through AssetCore): a Radiance RGBE (.hdr) equirectangular image,
+// resampled to the cube layout hh_scene_set_ibl takes.
+//   * RGBE (Ward, "Real Pixels", Graphics Gems II): value = (m + 0.5) *
+//     2^(e - 136) per channel, 0 when e = 0; scanlines flat or in the
+//     run-length form (2, 2, width) with each channel encoded separately;
+//     header FORMAT=32-bit_rle_rgbe, resolution "-Y H +X W" (top row first)
+//     or "+Y H +X W".
+//   * Lat-long layout: texel (i, j) of a W x H image holds direction
+//     (sin t sin p, cos t, -sin t cos p), t = pi (j + 0.5) / H,
+//     p = 2 pi ((i + 0.5) / W - 0.5); bilinear, repeating in i, clamped in j.
+//   * Cube faces in Vulkan layer order (+X, -X, +Y, -Y, +Z, -Z), texel
+//     (i, j) of face f at the direction of the samplers' face table; each
+//     texel averages k x k lat-long samples over its footprint, k =
+//     clamp(ceil(W / (4 size)), 1, 8).
+// ---------------------------------------------------------------------------
+namespace {
+
+bool read_whole(const char* path, std::vector<uint8_t>& out) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    uint8_t buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) out.insert(out.end(), buf, buf + n);
+    const bool ok = !std::ferror(f);
+    std::fclose(f);
+    return ok;
+}
+
+bool hdr_decode(const std::vector<uint8_t>& d, std::vector<float>& rgb, uint32_t& width, uint32_t& height,
+                std::string& err) {
+    size_t pos = 0;
+    auto line = [&](std::string& s) {
+        s.clear();
+        while (pos < d.size() && d[pos] != '\n') s += char(d[pos++]);
+        if (pos >= d.size()) return false;
+        pos++;
+        return true;
+    };
+    std::string s;
+    if (!line(s) || s.compare(0, 2, "#?") != 0) { err = "not a Radiance HDR file"; return false; }
+    bool format_ok = false;
+    for (;;) {
+        if (!line(s)) { err = "truncated header"; return false; }
+        if (s.empty()) break;
+        if (s.compare(0, 7, "FORMAT=") == 0) {
+            if (s != "FORMAT=32-bit_rle_rgbe") { err = "unsupported " + s; return false; }
+            format_ok = true;
+        }
+    }
+    if (!format_ok) { err = "missing FORMAT=32-bit_rle_rgbe"; return false; }
+    if (!line(s)) { err = "missing resolution line"; return false; }
+    char ya[3] = {0}, xa[3] = {0};
+    int h = 0, w = 0;
+    if (std::sscanf(s.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || std::string(xa) != "+X" ||
+        (std::string(ya) != "-Y" && std::string(ya) != "+Y"))
+        { err = "unsupported resolution line: " + s; return false; }
+    if (w <= 0 || h <= 0 || w > (1 << 15) || h > (1 << 15)) { err = "bad image size"; return false; }
+    const bool bottom_up = std::string(ya) == "+Y";
+    rgb.assign(size_t(w) * h * 3, 0.0f);
+    std::vector<uint8_t> row(size_t(w) * 4);
+    for (int r = 0; r < h; r++) {
+        if (w >= 8 && w < 32768 && pos + 4 <= d.size() && d[pos] == 2 && d[pos + 1] == 2 &&
+            ((int(d[pos + 2]) << 8) | d[pos + 3]) == w && !(d[pos + 2] & 0x80)) {
+            pos += 4;
+            for (int c = 0; c < 4; c++) {
+                int x = 0;
+                while (x < w) {
+                    if (pos >= d.size()) { err = "truncated scanline"; return false; }
+                    int count = d[pos++];
+                    if (count > 128) {
+                        count -= 128;
+                        if (count > w - x || pos >= d.size()) { err = "bad run length"; return false; }
+                        const uint8_t v = d[pos++];
+                        for (int k = 0; k < count; k++) row[size_t(x++) * 4 + c] = v;
+                    } else {
+                        if (count == 0 || count > w - x || pos + count > d.size()) { err = "bad literal run"; return false; }
+                        for (int k = 0; k < count; k++) row[size_t(x++) * 4 + c] = d[pos++];
+                    }
+                }
+            }
+        } else {
+            if (pos + row.size() > d.size()) { err = "truncated scanline"; return false; }
+            if (d[pos] == 1 && d[pos + 1] == 1 && d[pos + 2] == 1) { err = "old run-length scanlines are not supported"; return false; }
+            std::memcpy(row.data(), d.data() + pos, row.size());
+            pos += row.size();
+        }
+        const int dst = bottom_up ? h - 1 - r : r;
+        for (int x = 0; x < w; x++) {
+            const uint8_t* p = &row[size_t(x) * 4];
+            float* o = &rgb[(size_t(dst) * w + x) * 3];
+            if (p[3] == 0) continue;
+            const double f = std::ldexp(1.0, int(p[3]) - (128 + 8));
+            for (int c = 0; c < 3; c++) o[c] = float((p[c] + 0.5) * f);
+        }
+    }
+    width = uint32_t(w);
+    height = uint32_t(h);
+    return true;
+}
+
+void latlong_sample(const std::vector<float>& img, uint32_t w, uint32_t h, double dx, double dy, double dz, double out[3]) {
+    const double kPi = 3.14159265358979323846;
+    const double t = std::acos(std::max(-1.0, std::min(1.0, dy)));
+    const double p = std::atan2(dx, -dz);
+    const double x = (p / (2.0 * kPi) + 0.5) * w - 0.5, y = t / kPi * h - 0.5;
+    const double fx = std::floor(x), fy = std::floor(y);
+    const double ax = x - fx, ay = y - fy;
+    auto wrap = [w](long i) { long r = i % long(w); return uint32_t(r < 0 ? r + long(w) : r); };
+    auto clampy = [h](long j) { return uint32_t(j < 0 ? 0 : (j >= long(h) ? long(h) - 1 : j)); };
+    const uint32_t i0 = wrap(long(fx)), i1 = wrap(long(fx) + 1), j0 = clampy(long(fy)), j1 = clampy(long(fy) + 1);
+    for (int c = 0; c < 3; c++) {
+        const double a = img[(size_t(j0) * w + i0) * 3 + c], b = img[(size_t(j0) * w + i1) * 3 + c];
+        const double e = img[(size_t(j1) * w + i0) * 3 + c], f = img[(size_t(j1) * w + i1) * 3 + c];
+        out[c] = (a + (b - a) * ax) + ((e + (f - e) * ax) - (a + (b - a) * ax)) * ay;
+    }
+}
+
+}  // namespace
+
+extern "C" hh_status hh_ibl_from_hdr(const char* path, uint32_t size, float* out_faces) {
+    if (!path || !out_faces || size == 0 || size > 4096) return hh_internal_fail(HH_ERR_INVALID_ARG, "bad argument");
+    std::vector<uint8_t> bytes;
+    if (!read_whole(path, bytes)) return hh_internal_fail(HH_ERR_IO, (std::string("cannot read ") + path).c_str());
+    std::vector<float> img;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!hdr_decode(bytes, img, w, h, err)) return hh_internal_fail(HH_ERR_IO, (std::string(path) + ": " + err).c_str());
+    const int k = int(std::min<uint32_t>(8, std::max<uint32_t>(1, (w + 4 * size - 1) / (4 * size))));
+    for (int face = 0; face < 6; face++)
+        for (uint32_t j = 0; j < size; j++)
+            for (uint32_t i = 0; i < size; i++) {
+                double acc[3] = {0.0, 0.0, 0.0};
+                for (int sj = 0; sj < k; sj++)
+                    for (int si = 0; si < k; si++) {
+                        const double sc = 2.0 * ((i + (si + 0.5) / k) / size) - 1.0;
+                        const double tc = 2.0 * ((j + (sj + 0.5) / k) / size) - 1.0;
+                        double d[3];
+                        switch (face) {
+                            case 0: d[0] = 1.0; d[1] = -tc; d[2] = -sc; break;
+                            case 1: d[0] = -1.0; d[1] = -tc; d[2] = sc; break;
+                            case 2: d[0] = sc; d[1] = 1.0; d[2] = tc; break;
+                            case 3: d[0] = sc; d[1] = -1.0; d[2] = -tc; break;
+                            case 4: d[0] = sc; d[1] = -tc; d[2] = 1.0; break;
+                            default: d[0] = -sc; d[1] = -tc; d[2] = -1.0; break;
+                        }
+                        const double n = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                        double v[3];
+                        latlong_sample(img, w, h, d[0] / n, d[1] / n, d[2] / n, v);
+                        for (int c = 0; c < 3; c++) acc[c] += v[c];
+                    }
+                float* o = out_faces + ((size_t(face) * size + j) * size + i) * 4;
+                for (int c = 0; c < 3; c++) o[c] = float(acc[c] / (k * k));
+                o[3] = 1.0f;
+            }
+    return HH_OK;
+}
+
+extern "C" hh_status hh_scene_load_ibl_hdr(hh_scene scene, const char* path, uint32_t size) {
+    if (!scene || !path || size == 0 || size > 4096) return hh_internal_fail(HH_ERR_INVALID_ARG, "bad argument");
+    std::vector<float> faces(size_t(6) * size * size * 4);
+    const hh_status st = hh_ibl_from_hdr(path, size, faces.data());
+    if (st != HH_OK) return st;
+    return hh_scene_set_ibl(scene, size, faces.data());
 }

@@ -27,6 +27,36 @@ def forward_flops(h, w):
     macs = sum(9 * ci * co * (hp >> lv) * (wp >> lv) for ci, co, lv in CONVS) + 32 * 3 * hp * wp
     return 2 * macs
 
+# Stored tensors each convolution reads and writes: ([(channels, level)] in, (channels, level) out).
+# Inputs are BF16 (the first layer reads the 16-channel packed input); decoder
+# convolutions read the coarser level once (the nearest upsample is in the
+# loader) and the skip at their own level.
+_CONV_IO = [([(16, 0)], (32, 0)), ([(32, 0)], (32, 0)), ([(32, 1)], (48, 1)), ([(48, 1)], (48, 1)),
+            ([(48, 2)], (64, 2)), ([(64, 2)], (64, 2)), ([(64, 3)], (96, 3)), ([(96, 3)], (96, 3)),
+            ([(96, 3), (64, 2)], (64, 2)), ([(64, 2)], (64, 2)), ([(64, 2), (48, 1)], (48, 1)), ([(48, 1)], (48, 1)),
+            ([(48, 1), (32, 0)], (32, 0)), ([(32, 0)], (32, 0))]
+_POOLS = [(32, 0), (48, 1), (64, 2)]
+
+
+def forward_bytes(h, w):
+    """Algorithmic HBM bytes of one hh_denoise pass on an h x w surface (padded
+    to multiples of 8): every stored tensor read once and written once --
+    the packing (radiance + two feature planes in, 16 BF16 channels + the
+    FP32 residual out), the 14 convolutions, the three 2x2 poolings and the
+    head (32 channels + residual in, network output + surface out)."""
+    hp, wp = (h + 7) // 8 * 8, (w + 7) // 8 * 8
+
+    def px(lv):
+        return (hp >> lv) * (wp >> lv)
+
+    total = px(0) * (48 + 32 + 16)                                       # pack
+    for ins, (co, lo) in _CONV_IO:
+        total += sum(2 * c * px(lv) for c, lv in ins) + 2 * co * px(lo)
+    total += sum(2 * c * px(lv) + 2 * c * px(lv + 1) for c, lv in _POOLS)
+    total += px(0) * (2 * 32 + 16 + 16 + 16)                            # head
+    return total
+
+
 _SIGS = {
     "hh_denoiser_create": [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)],
     "hh_denoiser_destroy": [C.c_void_p],

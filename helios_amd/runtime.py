@@ -89,6 +89,8 @@ _SIGS = {
     "hh_context_reset_profile": [C.c_void_p],
     "hh_bandwidth_probe": [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)],
     "hh_scene_load_gltf": [C.c_void_p, C.c_char_p, C.POINTER(abi.GltfInfo)],
+    "hh_ibl_from_hdr": [C.c_char_p, C.c_uint32, C.c_void_p],
+    "hh_scene_load_ibl_hdr": [C.c_void_p, C.c_char_p, C.c_uint32],
     "hh_gltf_load": [C.c_char_p, C.POINTER(abi.SceneSink), C.POINTER(abi.GltfInfo)],
     "hh_render_output_buffer": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_void_p],
     "hh_surface_save_png": [C.c_void_p, C.c_float, C.c_uint32, C.c_char_p],
@@ -240,6 +242,10 @@ class Scene:
 
     def commit(self):
         check(lib().hh_scene_commit(self.h))
+
+    def load_ibl_hdr(self, path, size=512):
+        """IBL cube (size x size faces) from a Radiance .hdr lat-long image."""
+        check(lib().hh_scene_load_ibl_hdr(self.h, str(path).encode(), size))
 
     def load_gltf(self, path):
         """Add the contents of a glTF 2.0 file (.gltf / .glb); returns gltf_info
@@ -446,6 +452,14 @@ class PathIntegrator:
         if self.h:
             lib().hh_integrator_destroy(self.h)
             self.h = None
+
+
+def ibl_from_hdr(path, size):
+    """The six (size, size, 4) RGBA32F cube faces hh_scene_set_ibl takes, from
+    a Radiance .hdr lat-long image (host only)."""
+    out = np.zeros((6, size, size, 4), np.float32)
+    check(lib().hh_ibl_from_hdr(str(path).encode(), size, ptr(out)))
+    return out
 
 
 def _gltf_info(info):

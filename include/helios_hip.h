@@ -320,6 +320,17 @@ typedef struct hh_gltf_info {
 } hh_gltf_info;
 hh_status hh_gltf_load(const char* path, const hh_scene_sink* sink, hh_gltf_info* info);
 hh_status hh_scene_load_gltf(hh_scene scene, const char* path, hh_gltf_info* info);
+/* The IBL node's environment from a file (the reference loads a cube map
+ * through AssetCore, resource_manager.cpp:261-291,618-637): a Radiance RGBE (.hdr)
+ * equirectangular image (value = (m + 0.5) 2^(e - 136); flat or run-length
+ * scanlines; "-Y H +X W" or "+Y H +X W"), resampled to 6 faces of
+ * size x size RGBA32F in hh_scene_set_ibl's layout. Lat-long texel (i, j) of
+ * a W x H image holds direction (sin t sin p, cos t, -sin t cos p),
+ * t = pi (j + 0.5) / H, p = 2 pi ((i + 0.5) / W - 0.5); each cube texel
+ * averages k x k bilinear lat-long samples over its footprint,
+ * k = clamp(ceil(W / (4 size)), 1, 8). out_faces holds 6 size^2 4 floats. */
+hh_status hh_ibl_from_hdr(const char* path, uint32_t size, float* out_faces);
+hh_status hh_scene_load_ibl_hdr(hh_scene scene, const char* path, uint32_t size);
 
 /* ---------------------------------------------------------------------------
  * Output surface (replaces the RGBA32F ping-pong images)

@@ -45,3 +45,28 @@ def test_loaded_scene_renders_like_the_original(gpu_ctx, tmp_path, suffix):
     np.testing.assert_allclose(imgs[0][0], imgs[1][0], atol=5e-5)
     for va, vb in zip(imgs[0][1], imgs[1][1]):
         np.testing.assert_allclose(va, vb, atol=2e-6)
+
+
+def test_ibl_from_hdr_file(gpu_ctx, oracle_mod, tmp_path):
+    """hh_scene_load_ibl_hdr: the GPU scene lit by a .hdr environment renders
+    like the oracle given the faces hh_ibl_from_hdr produces."""
+    from helios_amd import runtime
+    from test_hdr_ibl import latlong_dirs, write_hdr
+    w, h = 256, 128
+    write_hdr(tmp_path / "env.hdr", 0.4 + 0.3 * latlong_dirs(w, h) + np.array([0.2, 0.0, -0.1]))
+    faces = runtime.ibl_from_hdr(tmp_path / "env.hdr", 32)
+    d = scenegen.cornell_box(width=40, height=32, max_ray_bounces=3)
+    d.ibl = (32, faces)
+    scene = runtime.Scene(gpu_ctx, None, commit=False)
+    blank = scenegen.cornell_box(width=40, height=32, max_ray_bounces=3)
+    scene.load(blank)
+    scene.load_ibl_hdr(tmp_path / "env.hdr", 32)
+    scene.commit()
+    assert np.array_equal(scene.env()[..., :4].reshape(faces.shape), faces)
+    surf = runtime.Surface(gpu_ctx, d.width, d.height)
+    it = runtime.PathIntegrator(gpu_ctx)
+    it.set_max_ray_bounces(3)
+    it.render(scene, d.camera, surf, frames=8)
+    ref = oracle_mod.OracleScene(d).render(frames=8, max_ray_bounces=3)
+    x, r = surf.read()[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    assert float(np.mean((x - r) ** 2 / (r * r + 0.01))) < 1e-4
