@@ -56,6 +56,7 @@ def parse():
     p.add_argument("--frames-in-flight", type=int, default=128)
     p.add_argument("--refill-lanes", type=int, default=32)
     p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
+    p.add_argument("--bvh-builder", type=int, default=1, help="0 LBVH on the GPU, 1 binned SAH on the host")
     p.add_argument("--megakernel", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -387,7 +388,7 @@ def main():
     ctx = runtime.Context(0)
     ctx.set_tuning(bvh_width=a.bvh_width, extend_block=a.block, persistent_blocks_per_cu=a.blocks_per_cu,
                    frames_in_flight=a.frames_in_flight, bvh_compressed=a.bvh_compressed,
-                   refill_lanes=a.refill_lanes, traversal=a.traversal)
+                   refill_lanes=a.refill_lanes, traversal=a.traversal, bvh_builder=a.bvh_builder)
     node_bytes = 64 if (a.bvh_compressed and a.bvh_width == 4) else NODE_BYTES[a.bvh_width]
     t0 = time.perf_counter()
     scene = runtime.Scene(ctx, desc)
@@ -455,6 +456,7 @@ def main():
             "spp_per_frame": 1, "frames_in_flight": a.frames_in_flight, "bvh_width": a.bvh_width,
             "bvh_compressed": bool(a.bvh_compressed), "block": a.block, "blocks_per_cu": a.blocks_per_cu,
             "refill_lanes": a.refill_lanes, "traversal": "if-if" if a.traversal else "while-while",
+            "bvh_builder": "binned SAH (host)" if a.bvh_builder else "LBVH (GPU)",
             "integrator": "megakernel" if a.megakernel else "wavefront+hipGraph",
         },
         "gsample_bounces_per_s": round(gsb, 4),

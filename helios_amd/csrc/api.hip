@@ -390,6 +390,8 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 256, "frames_in_flight must be 1..256");
     HH_ARG(t->refill_lanes >= 1 && t->refill_lanes <= 64, "refill_lanes must be 1..64");
     HH_ARG(t->traversal <= 1, "traversal must be 0 (while-while) or 1 (if-if)");
+    HH_ARG(t->bvh_builder <= 1, "bvh_builder must be 0 (LBVH) or 1 (SAH)");
+    ctx->tune.bvh_builder = t->bvh_builder;
     ctx->tune.refill_lanes = t->refill_lanes;
     ctx->tune.traversal = (int)t->traversal;
     ctx->tune.frames_in_flight = t->frames_in_flight;
@@ -494,6 +496,7 @@ hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t) {
     t->bvh_compressed = ctx->tune.bvh_compressed ? 1u : 0u;
     t->refill_lanes = ctx->tune.refill_lanes;
     t->traversal = (uint32_t)ctx->tune.traversal;
+    t->bvh_builder = ctx->tune.bvh_builder;
     return HH_OK;
 }
 
@@ -867,7 +870,8 @@ hh_status hh_scene_commit(hh_scene s) {
                                ctx->stream);
         uint32_t nn = 0;
         if (e == hipSuccess)
-            e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup, d_tris, d_shade_soup, d_shade, tri_base, d_nodes,
+            e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup, d_tris, d_shade_soup, d_shade,
+                           ctx->tune.bvh_builder == 1 ? soup.data() : nullptr, tri_base, d_nodes,
                            node_base, ctx->bs, &nn,
                            &blas_box[g * 6]);
         if (e != hipSuccess) {
@@ -919,7 +923,7 @@ hh_status hh_scene_commit(hh_scene s) {
         tlas_nodes = 1;
     } else {
         hipError_t e = build_tlas(ctx->stream, W, (uint32_t)s->instances.size(), ilo.data(), ihi.data(), d_nodes, node_base, ctx->bs,
-                                  &tlas_nodes);
+                                  &tlas_nodes, ctx->tune.bvh_builder == 1);
         if (e != hipSuccess) return hip_fail(e, "TLAS build");
     }
     s->bvh_stats[2] = tlas_nodes;

@@ -16,6 +16,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
+#include <thread>
 #include <vector>
 
 #include "gpu_scene.hpp"
@@ -372,18 +374,174 @@ inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 // references are offset by node_base (internal) and leaf_base (leaves).
 // On return *order_out (device, n entries) holds the primitive index of every
 // leaf slot and root_box the bounds of all primitives.
+// ---------------------------------------------------------------------------
+// Binned SAH build on the host: the quality a driver's PREFER_FAST_TRACE
+// build aims for (mesh.cpp:109), at the cost of a host pass per mesh. It
+// produces the same binary-tree form as the radix tree (internal nodes
+// 0..n-2, leaf j = binary ref n-1+j over the primitive order `order`,
+// contiguous ranges, parents), so the GPU refit and the wide collapse are
+// shared. Every split is chosen among 32 centroid bins on each axis by the
+// surface-area cost (leaf of m primitives costs m, a split costs its
+// children's area-weighted counts); ranges whose centroids coincide, or
+// whose best split would leave a side empty, are split at the median. The
+// subtree over m primitives owns the m - 1 internal ids after its root, so
+// subtrees are built in parallel without shared state.
+// ---------------------------------------------------------------------------
+namespace {
+struct SahPrim {
+    float lo[3], hi[3], c[3];
+};
+
+struct SahTree {
+    const std::vector<SahPrim>* prims;
+    std::vector<uint32_t> order;
+    std::vector<int2> childs, range;
+    std::vector<uint32_t> parent;
+    int n = 0;
+
+    static float area(const float lo[3], const float hi[3]) {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+
+    // split [b, e) and return the first index of the right side
+    int split(int b, int e) {
+        const std::vector<SahPrim>& P = *prims;
+        float cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = b; i < e; i++)
+            for (int k = 0; k < 3; k++) {
+                cl[k] = std::min(cl[k], P[order[i]].c[k]);
+                ch[k] = std::max(ch[k], P[order[i]].c[k]);
+            }
+        constexpr int NB = 32;
+        float best = INFINITY;
+        int best_axis = -1, best_bin = -1;
+        for (int k = 0; k < 3; k++) {
+            const float ext = ch[k] - cl[k];
+            if (!(ext > 0.0f)) continue;
+            const float scale = NB / ext;
+            int cnt[NB] = {0};
+            float blo[NB][3], bhi[NB][3];
+            for (int q = 0; q < NB; q++)
+                for (int a = 0; a < 3; a++) { blo[q][a] = INFINITY; bhi[q][a] = -INFINITY; }
+            for (int i = b; i < e; i++) {
+                const SahPrim& p = P[order[i]];
+                const int q = std::min(NB - 1, int((p.c[k] - cl[k]) * scale));
+                cnt[q]++;
+                for (int a = 0; a < 3; a++) {
+                    blo[q][a] = std::min(blo[q][a], p.lo[a]);
+                    bhi[q][a] = std::max(bhi[q][a], p.hi[a]);
+                }
+            }
+            float rarea[NB];
+            int rcnt[NB];
+            float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int rc = 0;
+            for (int q = NB - 1; q > 0; q--) {
+                rc += cnt[q];
+                for (int a = 0; a < 3; a++) { rl[a] = std::min(rl[a], blo[q][a]); rh[a] = std::max(rh[a], bhi[q][a]); }
+                rcnt[q] = rc;
+                rarea[q] = rc ? area(rl, rh) : 0.0f;
+            }
+            float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int lc = 0;
+            for (int q = 0; q < NB - 1; q++) {
+                lc += cnt[q];
+                for (int a = 0; a < 3; a++) { ll[a] = std::min(ll[a], blo[q][a]); lh[a] = std::max(lh[a], bhi[q][a]); }
+                if (lc == 0 || rcnt[q + 1] == 0) continue;
+                const float cost = lc * area(ll, lh) + rcnt[q + 1] * rarea[q + 1];
+                if (cost < best) { best = cost; best_axis = k; best_bin = q; }
+            }
+        }
+        if (best_axis >= 0) {
+            const float ext = ch[best_axis] - cl[best_axis], scale = NB / ext;
+            auto mid = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t i) {
+                return std::min(NB - 1, int((P[i].c[best_axis] - cl[best_axis]) * scale)) <= best_bin;
+            });
+            const int m = int(mid - order.begin());
+            if (m > b && m < e) return m;
+        }
+        // coincident centroids or a one-sided split: median on the widest axis
+        int ax = 0;
+        for (int k = 1; k < 3; k++)
+            if (ch[k] - cl[k] > ch[ax] - cl[ax]) ax = k;
+        const int m = b + (e - b) / 2;
+        std::nth_element(order.begin() + b, order.begin() + m, order.begin() + e,
+                         [&](uint32_t x, uint32_t y) { return P[x].c[ax] < P[y].c[ax]; });
+        return m;
+    }
+
+    int ref_of(int b, int e, int id) const { return e - b == 1 ? n - 1 + b : id; }
+
+    void build(int b, int e, int id, int depth) {
+        const int m = split(b, e);
+        const int lid = id + 1, rid = id + (m - b);
+        const int l = ref_of(b, m, lid), r = ref_of(m, e, rid);
+        childs[id] = make_int2(l, r);
+        range[id] = make_int2(b, e - 1);
+        parent[l] = uint32_t(id);
+        parent[r] = uint32_t(id);
+        const bool go_left = m - b > 1, go_right = e - m > 1;
+        if (go_left && go_right && depth < 5 && e - b > 65536) {
+            std::thread t([&, b, m, lid, depth] { build(b, m, lid, depth + 1); });
+            build(m, e, rid, depth + 1);
+            t.join();
+            return;
+        }
+        if (go_left) build(b, m, lid, depth + 1);
+        if (go_right) build(m, e, rid, depth + 1);
+    }
+};
+}  // namespace
+
+// Host tree over n primitive boxes (lo/hi, 3 floats each) -> order, childs,
+// range, parent in the radix tree's form.
+static void sah_tree(uint32_t n, const float* lo, const float* hi, SahTree& t, std::vector<SahPrim>& prims) {
+    prims.resize(n);
+    for (uint32_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            prims[i].lo[k] = lo[3 * i + k];
+            prims[i].hi[k] = hi[3 * i + k];
+            prims[i].c[k] = 0.5f * (lo[3 * i + k] + hi[3 * i + k]);
+        }
+    t.prims = &prims;
+    t.n = int(n);
+    t.order.resize(n);
+    for (uint32_t i = 0; i < n; i++) t.order[i] = i;
+    t.childs.assign(std::max<uint32_t>(n, 1), make_int2(0, 0));
+    t.range.assign(std::max<uint32_t>(n, 1), make_int2(0, 0));
+    t.parent.assign(2 * size_t(std::max<uint32_t>(n, 1)), 0u);
+    if (n > 1) t.build(0, int(n), 0, 0);
+}
+
 template <int W>
 static hipError_t build_wide(hipStream_t st, uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_cbounds,
                              WideNode<W>* d_nodes, uint32_t node_base, uint32_t leaf_base, int leaf_size, int leaf_is_prim,
-                             uint32_t* d_order, BuildScratch& sc, uint32_t* num_nodes, float root_box[6]) {
+                             uint32_t* d_order, BuildScratch& sc, uint32_t* num_nodes, float root_box[6],
+                             const float* h_lo = nullptr, const float* h_hi = nullptr) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
-    k_morton<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, n, d_cbounds, sc.keys_in, sc.vals_in);
-    size_t tb = sc.sort_bytes;
-    HH_TRY(hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.keys_in, sc.keys_out, sc.vals_in, d_order, (int)n, 0, 30, st));
-    if (n > 1) {
-        k_radix_tree<<<blocks(n - 1, B), B, 0, st>>>(sc.keys_out, (int)n, sc.childs, sc.parent, sc.range);
-        k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
+    if (h_lo) {
+        // binned SAH on the host (the boxes are the host copies of d_lo / d_hi)
+        SahTree t;
+        std::vector<SahPrim> prims;
+        sah_tree(n, h_lo, h_hi, t, prims);
+        HH_TRY(hipMemcpyAsync(d_order, t.order.data(), size_t(n) * 4, hipMemcpyHostToDevice, st));
+        if (n > 1) {
+            HH_TRY(hipMemcpyAsync(sc.childs, t.childs.data(), size_t(n - 1) * sizeof(int2), hipMemcpyHostToDevice, st));
+            HH_TRY(hipMemcpyAsync(sc.range, t.range.data(), size_t(n - 1) * sizeof(int2), hipMemcpyHostToDevice, st));
+            HH_TRY(hipMemcpyAsync(sc.parent, t.parent.data(), (2 * size_t(n) - 1) * 4, hipMemcpyHostToDevice, st));
+            k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
+        }
+        HH_TRY(hipStreamSynchronize(st));   // the host vectors go out of scope
+    } else {
+        k_morton<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, n, d_cbounds, sc.keys_in, sc.vals_in);
+        size_t tb = sc.sort_bytes;
+        HH_TRY(hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.keys_in, sc.keys_out, sc.vals_in, d_order, (int)n, 0, 30, st));
+        if (n > 1) {
+            k_radix_tree<<<blocks(n - 1, B), B, 0, st>>>(sc.keys_out, (int)n, sc.childs, sc.parent, sc.range);
+            k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
+        }
     }
     k_refit<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, d_order, (int)n, sc.childs, sc.parent, sc.flags, sc.blo, sc.bhi);
     int root = n > 1 ? 0 : 0;  // binary ref of the root (leaf 0 when n == 1: ref n-1+0 = 0)
@@ -459,20 +617,35 @@ void BuildScratch::release() {
 // writes the leaf-ordered triangles to d_tri_pool + tri_base and the nodes to
 // d_node_pool + node_base (global indices in child references).
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
-                      const float4* d_shade_src, float4* d_shade_pool, uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
+                      const float4* d_shade_src, float4* d_shade_pool, const float4* h_tri_src, uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
                       float root_box[6]) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
     k_init_bounds<<<1, 64, 0, st>>>(sc.cbounds);
     k_tri_aabbs<<<blocks(n, B), B, 0, st>>>(d_tri_src, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds);
     uint32_t* order = sc.vals_out_tmp(n);
+    std::vector<float> hlo, hhi;   // host triangle boxes for the SAH build
+    if (h_tri_src) {
+        hlo.resize(size_t(n) * 3);
+        hhi.resize(size_t(n) * 3);
+        for (uint32_t t = 0; t < n; t++) {
+            const float4 a = h_tri_src[3 * t], b = h_tri_src[3 * t + 1], c = h_tri_src[3 * t + 2];
+            const float va[3] = {a.x, a.y, a.z}, vb[3] = {b.x, b.y, b.z}, vc[3] = {c.x, c.y, c.z};
+            for (int k = 0; k < 3; k++) {
+                hlo[3 * t + k] = std::min(va[k], std::min(vb[k], vc[k]));
+                hhi[3 * t + k] = std::max(va[k], std::max(vb[k], vc[k]));
+            }
+        }
+    }
+    const float* pl = h_tri_src ? hlo.data() : nullptr;
+    const float* ph = h_tri_src ? hhi.data() : nullptr;
     hipError_t e;
     if (width == 8)
         e = build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base,
-                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box);
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph);
     else
         e = build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base,
-                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box);
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph);
     HH_TRY(e);
     k_gather_tris<<<blocks(n, B), B, 0, st>>>(d_tri_src, order, n, d_tri_pool + 3 * (size_t)tri_base);
     k_gather_shade<<<blocks(n, B), B, 0, st>>>(d_shade_src, order, n, d_shade_pool + 4 * (size_t)tri_base);
@@ -487,7 +660,7 @@ hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_
 
 // TLAS over instance AABBs (host arrays, n >= 1); leaves hold one instance.
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
-                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes) {
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah) {
     HH_TRY(sc.ensure(n));
     std::vector<float4> lo(n), hi(n);
     for (uint32_t i = 0; i < n; i++) {
@@ -501,11 +674,13 @@ hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, 
     k_box_centroids<<<blocks(n, B), B, 0, st>>>(sc.aabb_lo, sc.aabb_hi, n, sc.cbounds);
     uint32_t* order = sc.vals_out_tmp(n);
     float rb[6];
+    const float* pl = sah ? h_lo : nullptr;
+    const float* ph = sah ? h_hi : nullptr;
     if (width == 8)
         return build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base, 0, 1,
-                             1, order, sc, num_nodes, rb);
+                             1, order, sc, num_nodes, rb, pl, ph);
     return build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base, 0, 1, 1,
-                         order, sc, num_nodes, rb);
+                         order, sc, num_nodes, rb, pl, ph);
 }
 
 }  // namespace hh

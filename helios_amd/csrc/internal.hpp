@@ -31,11 +31,11 @@ struct BuildScratch {
 };
 
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
-                      const float4* d_shade_src, float4* d_shade_pool,
+                      const float4* d_shade_src, float4* d_shade_pool, const float4* h_tri_src,
                       uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
                       float root_box[6]);
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
-                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes);
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah);
 hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_t n);
 
 // Wavefront state of one render call (device buffers sized for max_paths).
@@ -81,6 +81,7 @@ struct Tuning {
     int leaf_size = 4;
     uint32_t frames_in_flight = 4;
     bool bvh_compressed = false;
+    uint32_t bvh_builder = 1;   // 0: LBVH on the GPU; 1: binned SAH on the host
     uint32_t refill_lanes = 32;  // idle lanes that make a traversal wave take new rays
     int traversal = 0;           // 0 while-while, 1 if-if step shape
     int layout() const { return bvh_compressed && bvh_width == 4 ? kLayoutQ4 : bvh_width; }

@@ -505,6 +505,11 @@ typedef struct hh_tuning {
      * the lane holds a leaf, then the leaf), 1 = if-if (one node or leaf per
      * step). Results are identical; only the schedule differs. */
     uint32_t traversal;
+    /* BLAS / TLAS builder, used at the next scene commit: 0 = LBVH on the GPU
+     * (Morton order, fast build), 1 = binned SAH on the host (fast trace,
+     * the role of PREFER_FAST_TRACE in mesh.cpp:109; the default). Hits are
+     * identical either way (ties resolve by instance, geometry, primitive). */
+    uint32_t bvh_builder;
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);

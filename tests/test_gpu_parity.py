@@ -206,10 +206,12 @@ def test_progressive_protocol(gpu_ctx, cornell, cornell_pair):
     assert np.array_equal(c, one)
 
 
-@pytest.mark.parametrize("layout", [dict(bvh_width=8), dict(bvh_width=4, bvh_compressed=True)])
+@pytest.mark.parametrize("layout", [dict(bvh_width=8), dict(bvh_width=4, bvh_compressed=True),
+                                    dict(bvh_builder=0), dict(bvh_builder=0, bvh_width=8)])
 def test_bvh_layouts_bit_identical(gpu_ctx, small_field, field_pair, oracle_mod, layout):
-    """BVH8 and the quantized BVH4 return the same hits as the oracle and the
-    same images as the float BVH4 (traversal results are order independent)."""
+    """BVH8, the quantized BVH4 and the LBVH trees return the same hits as
+    the oracle and the same images as the binned-SAH float BVH4 (traversal
+    results are independent of the tree's shape and visit order)."""
     from helios_amd import runtime
     g4, o = field_pair
     ref_img, _ = _render_gpu(gpu_ctx, g4, small_field, frames=3)
