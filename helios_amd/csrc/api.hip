@@ -1285,14 +1285,26 @@ static hipError_t ensure_tables(hh_context ctx, uint32_t nscenes, uint32_t nview
     return hipSuccess;
 }
 
+// Path slots for `total` paths and queue segments for every block size a
+// frame may be launched with (the profiled frames use 256-thread blocks).
+static hipError_t ensure_wavefront(hh_context ctx, uint32_t total) {
+    uint64_t qneed = 0;
+    for (int bs : {64, 128, 256}) {
+        const uint32_t g = frame_grid(total, ctx->num_cu, ctx->tune.blocks_per_cu, bs);
+        qneed = std::max<uint64_t>(qneed, queue_segment(total, g, (uint32_t)bs) * kQShards);
+    }
+    return ctx->wf.ensure(total, qneed);
+}
+
 static hh_status run_frame(hh_context ctx, const std::vector<SceneGPU>& scenes, const std::vector<ViewGPU>& views,
                            uint32_t total, uint32_t max_bounces, bool mega, const std::string& key_extra) {
     HH_HIP(ensure_tables(ctx, (uint32_t)scenes.size(), (uint32_t)views.size()));
     uint32_t cap_before = ctx->wf.capacity;
     uint32_t* spill_before = ctx->wf.spill;
-    HH_HIP(ctx->wf.ensure(total));
+    const uint64_t qcap_before = ctx->wf.qcap;
+    HH_HIP(ensure_wavefront(ctx, total));
     HH_HIP(ctx->wf.ensure_spill((size_t)ctx->num_cu * ctx->tune.blocks_per_cu * ctx->tune.extend_block));
-    if (ctx->wf.capacity != cap_before || ctx->wf.spill != spill_before) ctx->drop_graphs();
+    if (ctx->wf.capacity != cap_before || ctx->wf.qcap != qcap_before || ctx->wf.spill != spill_before) ctx->drop_graphs();
     HH_HIP(hipMemcpyAsync(ctx->d_scene_table, scenes.data(), scenes.size() * sizeof(SceneGPU), hipMemcpyHostToDevice, ctx->stream));
     HH_HIP(hipMemcpyAsync(ctx->d_views, views.data(), views.size() * sizeof(ViewGPU), hipMemcpyHostToDevice, ctx->stream));
     LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
@@ -1409,7 +1421,7 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
     // size the wavefront for the largest chunk first, so the statistics
     // counters are not reallocated after they are cleared
     const uint32_t max_chunk = std::max<uint32_t>(1, std::min<uint32_t>(frames, ctx->tune.frames_in_flight));
-    HH_HIP(ctx->wf.ensure(out->w * out->h * max_chunk));
+    HH_HIP(ensure_wavefront(ctx, out->w * out->h * max_chunk));
     unsigned long long zero[2] = {0, 0};
     HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
     const bool mega = (it->s.flags & HH_FLAG_MEGAKERNEL) != 0;
@@ -1531,7 +1543,7 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
     uint32_t chunk = std::max<uint32_t>(1, ctx->tune.frames_in_flight);
     if (ctx->tune.queue_capacity > 0)
         chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, (uint32_t)(ctx->tune.queue_capacity / std::max<size_t>(total_px, 1))));
-    HH_HIP(ctx->wf.ensure((uint32_t)std::min<size_t>(total_px * std::min(chunk, max_spp), 0xFFFFFFFFull)));
+    HH_HIP(ensure_wavefront(ctx, (uint32_t)std::min<size_t>(total_px * std::min(chunk, max_spp), 0xFFFFFFFFull)));
     HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
     for (uint32_t s = 0; s < max_spp; s += chunk) {
         std::vector<ViewGPU> vg;
@@ -2058,28 +2070,30 @@ hh_status hh_denoise(hh_denoiser d, hh_surface in, uint32_t num_frames, hh_surfa
 // ---------------------------------------------------------------------------
 // Wavefront buffers
 // ---------------------------------------------------------------------------
-hipError_t hh::Wavefront::ensure(uint32_t n) {
-    if (n <= capacity && T) return hipSuccess;
+hipError_t hh::Wavefront::ensure(uint32_t n, uint64_t queue_entries) {
+    if (n <= capacity && queue_entries <= qcap && T) return hipSuccess;
+    const size_t N = std::max<uint32_t>(std::max(n, capacity), 1024);
+    const size_t Q = std::max<uint64_t>(std::max(queue_entries, qcap), N);
     release();
-    size_t N = std::max<uint32_t>(n, 1024);
     HH_TRY(hipMalloc(&T, N * 16));
     HH_TRY(hipMalloc(&L, N * 16));
     HH_TRY(hipMalloc(&rng, N * 8));
     for (int k = 0; k < 2; k++) {
-        HH_TRY(hipMalloc(&q_o[k], N * 16));
-        HH_TRY(hipMalloc(&q_d[k], N * 16));
-        HH_TRY(hipMalloc(&q_slot[k], N * 4));
+        HH_TRY(hipMalloc(&q_o[k], Q * 16));
+        HH_TRY(hipMalloc(&q_d[k], Q * 16));
+        HH_TRY(hipMalloc(&q_slot[k], Q * 4));
     }
-    HH_TRY(hipMalloc(&hit, N * 16));
-    HH_TRY(hipMalloc(&hit_tri, N * 4));
-    HH_TRY(hipMalloc(&s_o, N * 16));
-    HH_TRY(hipMalloc(&s_d, N * 16));
-    HH_TRY(hipMalloc(&s_c, N * 16));
-    HH_TRY(hipMalloc(&s_slot, N * 4));
+    HH_TRY(hipMalloc(&hit, Q * 16));
+    HH_TRY(hipMalloc(&hit_tri, Q * 4));
+    HH_TRY(hipMalloc(&s_o, Q * 16));
+    HH_TRY(hipMalloc(&s_d, Q * 16));
+    HH_TRY(hipMalloc(&s_c, Q * 16));
+    HH_TRY(hipMalloc(&s_slot, Q * 4));
     HH_TRY(hipMalloc(&feat, N * 32));
-    HH_TRY(hipMalloc(&counters, 64));
+    HH_TRY(hipMalloc(&counters, (size_t)kCtrKinds * kQShards * kCtrStride * 4));
     HH_TRY(hipMalloc(&stats, 64));
     capacity = (uint32_t)N;
+    qcap = Q;
     return hipSuccess;
 }
 
@@ -2112,4 +2126,5 @@ void hh::Wavefront::release() {
     counters = nullptr;
     stats = nullptr;
     capacity = 0;
+    qcap = 0;
 }

@@ -45,6 +45,11 @@ __device__ inline uint32_t wave_alloc(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(below);
 }
 
+// Queue segment a wave allocates into and starts reading from (blocks are
+// whole waves).
+__device__ inline uint32_t wave_shard() { return ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kQShards; }
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
+
 __device__ inline uint32_t find_view(const ViewGPU* views, uint32_t nv, uint32_t slot) {
     uint32_t lo = 0, hi = nv - 1;
     while (lo < hi) {
@@ -374,7 +379,9 @@ template <int BS>
 __global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ views, uint32_t nv, uint32_t total,
                                                  float4* __restrict__ q_o, float4* __restrict__ q_d,
                                                  uint32_t* __restrict__ q_slot, float4* __restrict__ T, float4* __restrict__ L,
-                                                 uint2* __restrict__ rngs, uint32_t* __restrict__ counters) {
+                                                 uint2* __restrict__ rngs, uint32_t* __restrict__ counters, uint32_t seg) {
+    // The first extension queue is dealt out to the segments in 64-entry
+    // chunks: chunk c goes to segment c % kQShards at offset (c / kQShards) * 64.
     const uint32_t stride = gridDim.x * BS;
     for (uint32_t slot = blockIdx.x * BS + threadIdx.x; slot < total; slot += stride) {
         SlotInfo si = slot_info(views, nv, slot);
@@ -382,19 +389,24 @@ __global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ vie
         RNG rng = rng_init(si.x, si.y, v.first_frame + si.sample);
         f3 o, d;
         generate_ray(v, si.x, si.y, rng, o, d);
-        q_o[slot] = make_float4(o.x, o.y, o.z, 0.001f);
-        q_d[slot] = make_float4(d.x, d.y, d.z, 10000.0f);
-        q_slot[slot] = slot;
+        const uint32_t c = slot >> 6;
+        const uint32_t qi = (c % kQShards) * seg + (c / kQShards) * 64u + (slot & 63u);
+        q_o[qi] = make_float4(o.x, o.y, o.z, 0.001f);
+        q_d[qi] = make_float4(d.x, d.y, d.z, 10000.0f);
+        q_slot[qi] = slot;
         T[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
         L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         rngs[slot] = make_uint2(rng.s0, rng.s1);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[0] = total;
-        counters[1] = 0;
-        counters[2] = 0;
-        counters[4] = 0;
-        counters[5] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kQShards) {
+        const uint32_t sh = threadIdx.x, nch = (total + 63u) >> 6;
+        uint32_t items = (nch / kQShards + (sh < nch % kQShards ? 1u : 0u)) * 64u;
+        if (nch > 0u && (nch - 1u) % kQShards == sh) items -= nch * 64u - total;
+        counters[ctr_index(kCtrExt0, sh)] = items;
+        counters[ctr_index(kCtrExt1, sh)] = 0u;
+        counters[ctr_index(kCtrShadow, sh)] = 0u;
+        counters[ctr_index(kCtrExtHead, sh)] = 0u;
+        counters[ctr_index(kCtrShadowHead, sh)] = 0u;
     }
 }
 
@@ -415,32 +427,62 @@ __device__ inline void flush_counts(const TravCount& tc, unsigned long long* out
 
 // Persistent queue traversal: every wave keeps up to 64 rays in flight and,
 // whenever at least `refill` of its lanes have finished (or all have), claims
-// the next rays of the queue for them with one atomic on `head`. Lanes step
-// their rays independently, so a wave is not held by its longest ray.
-// (Claiming larger private chunks per atomic was measured slower: it starves
-// waves of work when the deep-bounce queues are short.)
+// the next rays for them with one atomic on the read head of a queue segment.
+// A wave starts on its own segment and moves on to the following ones when a
+// segment runs dry, so the claims of the grid spread over kQShards counters.
+// Lanes step their rays independently, so a wave is not held by its longest
+// ray. (Claiming larger private chunks per atomic was measured slower: it
+// starves waves of work when the deep-bounce queues are short.)
 struct QueueCursor {
-    uint32_t n, refill;
-    uint32_t* head;
-    bool drained;  // the queue is exhausted
-    __device__ inline QueueCursor(uint32_t n_, uint32_t refill_, uint32_t* head_)
-        : n(n_), refill(refill_), head(head_), drained(false) {}
-    // Queue index for this lane (>= n: none). Called by the whole wave in
-    // converged control flow, so lane 0 is active for the atomic.
+    const uint32_t* count;  // segment lengths, kCtrStride apart
+    uint32_t* head;         // segment read heads, kCtrStride apart
+    uint32_t seg, refill;
+    uint32_t shard, tried;  // wave-uniform
+    bool drained;           // every segment is exhausted
+    __device__ inline QueueCursor(const uint32_t* count_, uint32_t* head_, uint32_t seg_, uint32_t refill_)
+        : count(count_), head(head_), seg(seg_), refill(refill_), shard(wave_shard()), tried(0u), drained(false) {}
+    // Queue entry for this lane (kNoEntry: none). Called by the whole wave in
+    // converged control flow, so lane 0 is active for the atomics.
     __device__ inline uint32_t next(bool idle_lane) {
-        if (drained) return 0xFFFFFFFFu;
+        if (drained) return kNoEntry;
         const unsigned long long idle = __ballot(idle_lane);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (nidle < refill && nidle != 64u) return 0xFFFFFFFFu;
-        uint32_t base = 0;
-        if (__lane_id() == 0) base = atomicAdd(head, nidle);
-        base = __builtin_amdgcn_readfirstlane(base);
-        drained = base + nidle >= n;
+        if (nidle < refill && nidle != 64u) return kNoEntry;
         const uint32_t lane = __lane_id();
         const unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
-        return idle_lane ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
+        const uint32_t rank = (uint32_t)__popcll(below);
+        uint32_t got = kNoEntry, done = 0u;
+        while (done < nidle) {
+            const uint32_t want = nidle - done;
+            const uint32_t n = count[shard * kCtrStride];
+            uint32_t base = 0;
+            if (lane == 0) {
+                // a segment already read to its end costs no atomic
+                base = __hip_atomic_load(head + shard * kCtrStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (base < n) base = atomicAdd(head + shard * kCtrStride, want);
+            }
+            base = __builtin_amdgcn_readfirstlane(base);
+            const uint32_t avail = base < n ? min(n - base, want) : 0u;
+            if (idle_lane && rank >= done && rank < done + avail) got = shard * seg + base + (rank - done);
+            done += avail;
+            if (base + want >= n) {
+                shard = shard + 1u == kQShards ? 0u : shard + 1u;
+                if (++tried == kQShards) {
+                    drained = true;
+                    break;
+                }
+            }
+        }
+        return got;
     }
 };
+
+// Sum of the kQShards segment lengths.
+__device__ inline uint32_t queue_total(const uint32_t* count) {
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < kQShards; k++) n += count[k * kCtrStride];
+    return n;
+}
 
 // Six waves per SIMD: the register budget (<= 80 VGPRs) that lets the LDS
 // stacks of 24 waves share one CU; measured faster than five waves with no
@@ -448,23 +490,22 @@ struct QueueCursor {
 template <int LY, int BS, int MODE, bool COUNT>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ q_o, const float4* __restrict__ q_d,
-                                               const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count_p,
-                                               uint32_t* __restrict__ head, uint32_t refill, uint32_t rflags,
+                                               const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count,
+                                               uint32_t* __restrict__ head, uint32_t seg, uint32_t refill, uint32_t rflags,
                                                float4* __restrict__ hit, uint32_t* __restrict__ hit_tri,
                                                unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
                                                uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
-    const uint32_t n = *count_p;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)n);
-    QueueCursor qc(n, refill, head);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)queue_total(count));
+    QueueCursor qc(count, head, seg, refill);
     TravCount tc{0u, 0u, 0u};
     Tracer<LY, false, COUNT> tr;
     uint32_t q = 0;
     bool active = false;
     for (;;) {
         const uint32_t nq = qc.next(!active);
-        if (nq < n) {
+        if (nq != kNoEntry) {
             const uint32_t slot = q_slot[nq];
             const uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
             const float4 o = q_o[nq], d = q_d[nq];
@@ -486,20 +527,44 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_
 template <int BS>
 __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                               uint32_t depth, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
-                                              const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count_p,
-                                              const float4* __restrict__ hit, const uint32_t* __restrict__ hit_tri,
+                                              const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count,
+                                              uint32_t seg, const float4* __restrict__ hit, const uint32_t* __restrict__ hit_tri,
                                               float4* __restrict__ T, float4* __restrict__ L, uint2* __restrict__ rngs,
                                               float4* __restrict__ n_o, float4* __restrict__ n_d, uint32_t* __restrict__ n_slot,
                                               uint32_t* __restrict__ n_count, float4* __restrict__ s_o, float4* __restrict__ s_d,
                                               float4* __restrict__ s_c, uint32_t* __restrict__ s_slot,
                                               uint32_t* __restrict__ s_count, float4* __restrict__ feat, uint32_t feat_stride) {
-    const uint32_t n = *count_p;
+    // The input queue is read as one list over its segments: entry q lives in
+    // segment k with first[k] <= q < first[k] + count[k] (first = exclusive
+    // prefix sum of the lengths, held in lanes 0..kQShards-1).
+    const uint32_t lane = __lane_id();
+    const uint32_t len = lane < kQShards ? count[lane * kCtrStride] : 0u;
+    uint32_t incl = len;
+    for (uint32_t d = 1; d < kQShards; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t first = incl - len;
+    const uint32_t n = __shfl(incl, (int)kQShards - 1);
+    const uint32_t out_shard = wave_shard();
+    uint32_t* const n_ctr = n_count + out_shard * kCtrStride;
+    uint32_t* const s_ctr = s_count + out_shard * kCtrStride;
     const uint32_t stride = gridDim.x * BS;
     // Keep whole waves in the loop so the ballot-based compaction sees every
     // active lane: iterate over the count rounded up to the grid stride.
     const uint32_t n_round = ((n + stride - 1) / stride) * stride;
-    for (uint32_t q = blockIdx.x * BS + threadIdx.x; q < n_round; q += stride) {
-        const bool active = q < n;
+    for (uint32_t ql = blockIdx.x * BS + threadIdx.x; ql < n_round; ql += stride) {
+        const bool active = ql < n;
+        uint32_t q = ql, qk = 0;
+#pragma unroll
+        for (uint32_t k = 1; k < kQShards; k++) {
+            const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)first, (int)k);
+            if (ql >= f) {
+                qk = k;
+                q = ql - f;
+            }
+        }
+        q += qk * seg;
         bool want_ext = false, want_sh = false;
         f3 ext_o = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 1.f);
         ShadowRay sr;
@@ -554,13 +619,13 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
             L[slot] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
             want_sh = sr.want;
         }
-        uint32_t ei = wave_alloc(n_count, want_ext);
+        const uint32_t ei = out_shard * seg + wave_alloc(n_ctr, want_ext);
         if (want_ext) {
             n_o[ei] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0001f);
             n_d[ei] = make_float4(ext_d.x, ext_d.y, ext_d.z, 10000.0f);
             n_slot[ei] = slot;
         }
-        uint32_t si = wave_alloc(s_count, want_sh);
+        const uint32_t si = out_shard * seg + wave_alloc(s_ctr, want_sh);
         if (want_sh) {
             s_o[si] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmin);
             s_d[si] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.tmax);
@@ -574,22 +639,21 @@ template <int LY, int BS, int MODE, bool COUNT>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_shadow(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ s_o, const float4* __restrict__ s_d,
                                                const float4* __restrict__ s_c, const uint32_t* __restrict__ s_slot,
-                                               const uint32_t* __restrict__ count_p, uint32_t* __restrict__ head,
-                                               uint32_t refill, float4* __restrict__ L,
+                                               const uint32_t* __restrict__ count, uint32_t* __restrict__ head,
+                                               uint32_t seg, uint32_t refill, float4* __restrict__ L,
                                                unsigned long long* __restrict__ stats, unsigned long long* __restrict__ tcounts,
                                                uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
-    const uint32_t n = *count_p;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], (unsigned long long)n);
-    QueueCursor qc(n, refill, head);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[1], (unsigned long long)queue_total(count));
+    QueueCursor qc(count, head, seg, refill);
     TravCount tc{0u, 0u, 0u};
     Tracer<LY, true, COUNT> tr;
     uint32_t q = 0;
     bool active = false;
     for (;;) {
         const uint32_t nq = qc.next(!active);
-        if (nq < n) {
+        if (nq != kNoEntry) {
             const uint32_t slot = s_slot[nq];
             const uint32_t view = nv == 1 ? 0u : find_view(views, nv, slot);
             const float4 o = s_o[nq], d = s_d[nq];
@@ -612,12 +676,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_
     if (COUNT) flush_counts(tc, tcounts + 3);
 }
 
-__global__ void k_reset4(uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* d) {
-    if (threadIdx.x == 0) {
-        *a = 0;
-        *b = 0;
-        *c = 0;
-        *d = 0;
+// Clears the next extension queue, the shadow queue and both read heads.
+__global__ void k_reset_queues(uint32_t* counters, uint32_t next_ext) {
+    const uint32_t sh = threadIdx.x;
+    if (sh < kQShards) {
+        counters[ctr_index(next_ext, sh)] = 0u;
+        counters[ctr_index(kCtrShadow, sh)] = 0u;
+        counters[ctr_index(kCtrExtHead, sh)] = 0u;
+        counters[ctr_index(kCtrShadowHead, sh)] = 0u;
     }
 }
 
@@ -1073,46 +1139,54 @@ struct ProfScope {
     }
 };
 
-// counters: [0], [1] extension queue lengths (ping-pong), [2] shadow queue
-// length, [4] extension queue head, [5] shadow queue head.
+// Queue counters: see ctr_index (extension queue lengths ping-pong between
+// kCtrExt0 and kCtrExt1; each kind has kQShards segment counters).
 template <int W, int BS, int MODE, bool COUNT>
 static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, const ViewGPU* d_views, const ViewGPU* h_views,
                              uint32_t nv, uint32_t total, uint32_t max_bounces, Wavefront& wf, bool mega) {
     hipStream_t st = cfg.stream;
     Prof* pf = cfg.prof;
     unsigned long long* tc = pf ? pf->d_counts : nullptr;
-    const uint32_t grid_cap = (uint32_t)cfg.num_cu * (uint32_t)cfg.tune.blocks_per_cu;
-    const uint32_t grid = std::max(1u, std::min(grid_cap, cdiv(total, BS)));
+    const uint32_t grid = frame_grid(total, cfg.num_cu, cfg.tune.blocks_per_cu, BS);
     const uint32_t refill = cfg.tune.refill_lanes;
     uint32_t* ctr = wf.counters;
+    const uint64_t seg64 = queue_segment(total, grid, BS);
+    if (!mega && (seg64 * kQShards > wf.qcap || seg64 * kQShards > 0xFFFFFFFFull)) return hipErrorInvalidValue;
+    const uint32_t seg = (uint32_t)seg64;
     if (mega) {
         ProfScope ps(pf, PK_MEGAKERNEL, st);
         k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, total, wf.L, wf.stats, wf.spill, wf.feat, wf.capacity);
     } else {
         {
             ProfScope ps(pf, PK_GENERATE, st);
-            k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, ctr);
+            k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, ctr,
+                                                seg);
         }
         for (uint32_t d = 0; d < max_bounces; d++) {
             const int a = d & 1, b = a ^ 1;
             {
                 ProfScope ps(pf, PK_EXTEND, st);
                 k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
-                                                                  ctr + a, ctr + 4, refill,
+                                                                  ctr + ctr_index(kCtrExt0 + a, 0),
+                                                                  ctr + ctr_index(kCtrExtHead, 0), seg, refill,
                                                                   d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
                                                                   wf.hit_tri, wf.stats, tc, wf.spill);
             }
-            k_reset4<<<1, 64, 0, st>>>(ctr + b, ctr + 2, ctr + 4, ctr + 5);
+            k_reset_queues<<<1, 64, 0, st>>>(ctr, kCtrExt0 + b);
             {
                 ProfScope ps(pf, PK_SHADE, st);
-                k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a], ctr + a,
+                k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
+                                                 ctr + ctr_index(kCtrExt0 + a, 0), seg,
                                                  wf.hit, wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b],
-                                                 ctr + b, wf.s_o, wf.s_d, wf.s_c, wf.s_slot, ctr + 2, wf.feat, wf.capacity);
+                                                 ctr + ctr_index(kCtrExt0 + b, 0), wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+                                                 ctr + ctr_index(kCtrShadow, 0), wf.feat, wf.capacity);
             }
             {
                 ProfScope ps(pf, PK_SHADOW, st);
                 k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
-                                                                  ctr + 2, ctr + 5, refill, wf.L, wf.stats, tc, wf.spill);
+                                                                  ctr + ctr_index(kCtrShadow, 0),
+                                                                  ctr + ctr_index(kCtrShadowHead, 0), seg, refill, wf.L,
+                                                                  wf.stats, tc, wf.spill);
             }
         }
     }

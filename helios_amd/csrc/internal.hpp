@@ -38,9 +38,39 @@ hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, 
                       uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah);
 hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_t n);
 
+// Ray queues are split into kQShards segments, each with its own length and
+// read-head counter on its own 256-B line. Same-address atomics are executed
+// one at a time at the memory side (about 90 M per second on MI355X, growing
+// linearly with the number of distinct counters), so one counter per queue
+// would serialize the per-wave allocations and refills of a whole frame.
+constexpr uint32_t kQShards = 16;
+constexpr uint32_t kCtrStride = 64;  // u32 words between two counters
+enum CtrKind : uint32_t { kCtrExt0 = 0, kCtrExt1 = 1, kCtrShadow = 2, kCtrExtHead = 3, kCtrShadowHead = 4, kCtrKinds = 5 };
+__host__ __device__ inline uint32_t ctr_index(uint32_t kind, uint32_t shard) { return (kind * kQShards + shard) * kCtrStride; }
+
+// Entries per queue segment for `total` paths traced by a persistent grid of
+// `grid` blocks of `bs` threads. A producer wave w writes to segment w % kQShards
+// and emits at most one extension and one shadow ray per queue entry it reads;
+// with grid-stride consumption every wave reads at most ceil(total / stride)
+// entries per lane. The initial queue is dealt out in 64-entry chunks.
+inline uint64_t queue_segment(uint32_t total, uint32_t grid, uint32_t bs) {
+    const uint64_t stride = (uint64_t)grid * bs, nwaves = stride / 64;
+    const uint64_t per_wave = ((total + stride - 1) / stride) * 64;
+    const uint64_t shade = ((nwaves + kQShards - 1) / kQShards) * per_wave;
+    const uint64_t chunks = ((uint64_t)total + 63) / 64;
+    const uint64_t gen = ((chunks + kQShards - 1) / kQShards) * 64;
+    return shade > gen ? shade : gen;
+}
+inline uint32_t frame_grid(uint32_t total, int num_cu, int blocks_per_cu, int bs) {
+    const uint32_t cap = (uint32_t)num_cu * (uint32_t)blocks_per_cu;
+    const uint32_t need = (total + (uint32_t)bs - 1) / (uint32_t)bs;
+    return need < 1 ? 1u : (need < cap ? need : cap);
+}
+
 // Wavefront state of one render call (device buffers sized for max_paths).
 struct Wavefront {
-    uint32_t capacity = 0;
+    uint32_t capacity = 0;   // path slots (T, L, rng, feat)
+    uint64_t qcap = 0;       // entries of each queue (kQShards segments)
     // path state by slot
     float4* T = nullptr;
     float4* L = nullptr;
@@ -60,13 +90,13 @@ struct Wavefront {
     // first-hit features by slot, written at depth 0 when a view has feature
     // planes: feat[slot] = (albedo, 1/(1+t)), feat[capacity + slot] = (normal, 0)
     float4* feat = nullptr;
-    // counters: [0,1] extension queue sizes, [2] shadow size, [3..] stats
+    // kCtrKinds x kQShards counters, see ctr_index
     uint32_t* counters = nullptr;
     unsigned long long* stats = nullptr;  // [0] extension rays, [1] shadow rays
     // traversal stack overflow, kSpillStackDepth words per launched thread
     uint32_t* spill = nullptr;
     size_t spill_threads = 0;
-    hipError_t ensure(uint32_t n);
+    hipError_t ensure(uint32_t n, uint64_t queue_entries);
     hipError_t ensure_spill(size_t threads);
     void release();
     ~Wavefront() { release(); }
