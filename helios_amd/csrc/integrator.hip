@@ -33,17 +33,30 @@ namespace hh {
 // ---------------------------------------------------------------------------
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated queue allocation: one atomic per wave.
-__device__ inline uint32_t wave_alloc(uint32_t* counter, bool want) {
-    unsigned long long m = __ballot(want);
+// Wave-aggregated allocation that orders the wave's entries by a small key
+// (lanes with a smaller key first, then by lane): one atomic per wave plus
+// NK ballots. Used to place rays of the same direction octant next to each
+// other, so the lanes a traversal wave refills at once start coherent.
+template <uint32_t NK>
+__device__ inline uint32_t wave_alloc_keyed(uint32_t* counter, bool want, uint32_t key) {
+    const unsigned long long m = __ballot(want);
     if (m == 0ull) return 0u;
-    uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+    const uint32_t lane = lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < NK; k++) {
+        const unsigned long long mk = __ballot(want && key == k);
+        if (k < key) pos += (uint32_t)__popcll(mk);
+        else if (k == key) pos += (uint32_t)__popcll(mk & below);
+    }
+    const uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
     uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
     base = __shfl(base, (int)leader);
-    unsigned long long below = (lane_id() == 0) ? 0ull : (m & ((1ull << lane_id()) - 1ull));
-    return base + (uint32_t)__popcll(below);
+    return base + pos;
 }
+__device__ inline uint32_t octant(f3 d) { return (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u); }
 
 // Queue segment a wave allocates into and starts reading from (blocks are
 // whole waves).
@@ -63,14 +76,17 @@ __device__ inline uint32_t find_view(const ViewGPU* views, uint32_t nv, uint32_t
 struct SlotInfo {
     uint32_t view, x, y, sample;
 };
+// Path slots of a view are numbered pixel-major: the spp samples of one pixel
+// are consecutive, so the rays of a wave (64 consecutive slots of the first
+// queue) start from the same or neighbouring pixels and traverse the same
+// nodes. Results do not depend on the numbering.
 __device__ inline SlotInfo slot_info(const ViewGPU* views, uint32_t nv, uint32_t slot) {
     SlotInfo si;
     si.view = nv == 1 ? 0u : find_view(views, nv, slot);
     const ViewGPU& v = views[si.view];
     uint32_t local = slot - v.slot_base;
-    uint32_t npix = v.tile_w * v.tile_h;
-    si.sample = local / npix;
-    uint32_t p = local - si.sample * npix;
+    uint32_t p = local / v.spp;
+    si.sample = local - p * v.spp;
     si.y = v.tile_y + p / v.tile_w;
     si.x = v.tile_x + p % v.tile_w;
     return si;
@@ -619,13 +635,13 @@ __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scene
             L[slot] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
             want_sh = sr.want;
         }
-        const uint32_t ei = out_shard * seg + wave_alloc(n_ctr, want_ext);
+        const uint32_t ei = out_shard * seg + wave_alloc_keyed<8>(n_ctr, want_ext, octant(ext_d));
         if (want_ext) {
             n_o[ei] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0001f);
             n_d[ei] = make_float4(ext_d.x, ext_d.y, ext_d.z, 10000.0f);
             n_slot[ei] = slot;
         }
-        const uint32_t si = out_shard * seg + wave_alloc(s_ctr, want_sh);
+        const uint32_t si = out_shard * seg + wave_alloc_keyed<8>(s_ctr, want_sh, octant(sr.d));
         if (want_sh) {
             s_o[si] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmin);
             s_d[si] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.tmax);
@@ -707,7 +723,7 @@ __global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, con
             a1 = aux[(size_t)v.width * v.height + pi];
         }
         for (uint32_t s = 0; s < v.spp; s++) {
-            const uint32_t slot = v.slot_base + s * npix + p;
+            const uint32_t slot = v.slot_base + p * v.spp + s;
             float4 l = L[slot];
             float3 c;
             if (v.flags & HH_FLAG_NO_CLAMP_BIT) c = make_float3(l.x, l.y, l.z);

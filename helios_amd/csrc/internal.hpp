@@ -43,7 +43,7 @@ hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_
 // one at a time at the memory side (about 90 M per second on MI355X, growing
 // linearly with the number of distinct counters), so one counter per queue
 // would serialize the per-wave allocations and refills of a whole frame.
-constexpr uint32_t kQShards = 16;
+constexpr uint32_t kQShards = 32;
 constexpr uint32_t kCtrStride = 64;  // u32 words between two counters
 enum CtrKind : uint32_t { kCtrExt0 = 0, kCtrExt1 = 1, kCtrShadow = 2, kCtrExtHead = 3, kCtrShadowHead = 4, kCtrKinds = 5 };
 __host__ __device__ inline uint32_t ctr_index(uint32_t kind, uint32_t shard) { return (kind * kQShards + shard) * kCtrStride; }
