@@ -724,7 +724,13 @@ hh_status hh_scene_commit(hh_scene s) {
             }
         }
         submesh_off.push_back((uint32_t)submesh_info.size());
-        for (const hh_submesh& sm : me.sm) submesh_info.push_back(make_uint2(sm.base_index / 3, (uint32_t)mat_global[material_of(sm)]));
+        // scene.cpp:934,1143: an unordered_map lookup, so a material that was
+        // never packed (the mesh was first packed under another instance's
+        // override) reads as material 0
+        for (const hh_submesh& sm : me.sm) {
+            const int gm = mat_global[material_of(sm)];
+            submesh_info.push_back(make_uint2(sm.base_index / 3, gm < 0 ? 0u : (uint32_t)gm));
+        }
         InstanceData id{};
         std::memcpy(id.model, inst.model, 64);
         std::memcpy(id.normal, inst.normal, 64);
@@ -899,6 +905,26 @@ hh_status hh_scene_commit(hh_scene s) {
             for (int c = 0; c < 4; c++) trav[i].w2o[r * 4 + c] = (float)inv.m[c][r];
         trav[i].blas_root = blas_root[g];
         trav[i].mesh = g;
+        // one alpha source for all non-opaque geometry of the instance, if any
+        // (the geometry flag follows the mesh's material, the alpha the
+        // instance's, as the any-hit test does)
+        {
+            const MeshHost& me = s->meshes[inst.mesh];
+            uint32_t src = kAlphaConst, val = 0;
+            bool first = true;
+            for (uint32_t gi = 0; gi < me.sm.size(); gi++) {
+                const hh_material_desc& own = s->materials[me.mats[me.sm[gi].mat_idx]];
+                if (own.type == HH_MATERIAL_OPAQUE && !own.alpha_test) continue;
+                const MaterialData& md = mat_data[submesh_info[submesh_off[i] + gi].y];
+                uint32_t gsrc = kAlphaConst, gval = 0;
+                if (md.tex0[0] != -1) gsrc = (uint32_t)md.tex0[0];
+                else std::memcpy(&gval, &md.albedo[3], 4);
+                if (first) { src = gsrc; val = gval; first = false; }
+                else if (gsrc != src || gval != val) { src = kAlphaMixed; val = 0; break; }
+            }
+            trav[i].alpha_src = src;
+            trav[i].alpha_value = val;
+        }
         const float* bb = &blas_box[g * 6];
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int c = 0; c < 8; c++) {

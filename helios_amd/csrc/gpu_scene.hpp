@@ -79,8 +79,15 @@ struct alignas(16) InstTrav {
     float w2o[12];        // row-major 3x4 world->object
     uint32_t blas_root;   // global node index
     uint32_t mesh;        // global mesh index (reference mesh_index)
-    uint32_t pad[2];
+    // Alpha of a candidate hit on this instance's non-opaque geometry when all
+    // of it uses one alpha source: a texture slot (the albedo texture, alpha
+    // channel), kAlphaConst (alpha = __uint_as_float(alpha_value)) or
+    // kAlphaMixed (look the material up per geometry).
+    uint32_t alpha_src;
+    uint32_t alpha_value;
 };
+constexpr uint32_t kAlphaConst = 0xFFFFFFFEu;
+constexpr uint32_t kAlphaMixed = 0xFFFFFFFFu;
 
 // MaterialData (scene.cpp:25-32)
 struct alignas(16) MaterialData {

@@ -160,15 +160,23 @@ __device__ inline float4 sample_tex(const SceneGPU& S, int slot, float s, float 
 
 // path_trace_rahit.glsl:162-188: alpha of the candidate hit on triangle
 // `tri` (global index; its UVs come from the shading record).
+// The instance record names the alpha source of its non-opaque geometry when
+// it is the same for all of it, which saves the submesh and material lookups.
 __device__ inline float hit_alpha(const SceneGPU& S, uint32_t inst, uint32_t geom, uint32_t tri, float u, float v) {
-    uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
-    const MaterialData& m = S.mats[info.y];
-    if (m.tex0[0] == -1) return m.albedo[3];
+    const uint2 src = *reinterpret_cast<const uint2*>(&S.inst_trav[inst].alpha_src);
     const float4 r0 = S.shade[4 * (size_t)tri], r1 = S.shade[4 * (size_t)tri + 1];
+    int slot = (int)src.x;
+    if (src.x == kAlphaConst) return __uint_as_float(src.y);
+    if (src.x == kAlphaMixed) {
+        const uint2 info = S.submesh_info[S.submesh_off[inst] + geom];
+        const MaterialData& m = S.mats[info.y];
+        if (m.tex0[0] == -1) return m.albedo[3];
+        slot = m.tex0[0];
+    }
     float bx = (1.0f - u) - v, by = u, bz = v;
     float tu = (r0.x * bx + r0.z * by) + r1.x * bz;
     float tv = (r0.y * bx + r0.w * by) + r1.y * bz;
-    return sample_tex(S, m.tex0[0], tu, tv).w;
+    return sample_tex(S, slot, tu, tv).w;
 }
 
 // One slab test in the robust form (Ize 2013): returns the entry distance or

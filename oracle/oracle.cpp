@@ -490,7 +490,11 @@ void commit(Scene& s) {
         std::vector<uint32_t> info;
         for (const hh_submesh& sm : me.submeshes) {
             info.push_back(sm.base_index / 3);
-            info.push_back((uint32_t)mat_global[material_of(sm)]);
+            // scene.cpp:934,1143: global_material_indices is an unordered_map, so
+            // a material that was never packed (its mesh was first packed under
+            // another node's override) reads as material 0 (Q19)
+            const int gm = mat_global[material_of(sm)];
+            info.push_back(gm < 0 ? 0u : (uint32_t)gm);
         }
         s.submesh_info.push_back(info);
         InstanceData id{};

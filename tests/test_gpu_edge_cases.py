@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from helios_amd import scenegen
-from helios_amd.abi import LIGHT_DIRECTIONAL
+from helios_amd.abi import LIGHT_DIRECTIONAL, LIGHT_POINT
 
 pytestmark = pytest.mark.gpu
 
@@ -128,3 +128,55 @@ def test_instance_limit(gpu_ctx):
         d.instances.append(d.instances[0])
         extra.load(d)
     assert e.value.status == 3
+
+
+def _quad(x0, x1, y0, y1, z):
+    V = np.array([[x0, y0, z], [x1, y0, z], [x1, y1, z], [x0, y1, z]], np.float64)
+    N = np.tile([0.0, 0.0, 1.0], (4, 1))
+    U = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float64)
+    return V, N, U, np.array([0, 1, 2, 0, 2, 3], np.uint32)
+
+
+def test_alpha_sources(gpu_ctx, oracle_mod):
+    """The any-hit alpha test reads one alpha source per instance when all its
+    non-opaque geometry shares it (a texture slot or a constant alpha) and
+    looks the material up per geometry otherwise; a material override of the
+    instance supplies the alpha while the mesh's own material decides which
+    geometry is tested. Every case returns the oracle's hits."""
+    from helios_amd import runtime
+    d = scenegen.SceneDesc(width=128, height=64, max_ray_bounces=2)
+    tex = scenegen.leaf_texture(size=32)
+    d.textures.append((tex.shape[1], tex.shape[0], scenegen.TEXTURE_RGBA8_SRGB, tex))
+    d.materials += [scenegen.material((0.5, 0.5, 0.5, 1.0)),                                # 0 opaque
+                    scenegen.material((1, 1, 1, 1), albedo_texture=0, alpha_test=True),     # 1 textured alpha
+                    scenegen.material((1, 1, 1, 0.05), alpha_test=True),                   # 2 constant, cut away
+                    scenegen.material((1, 1, 1, 1.0), alpha_test=True)]                    # 3 constant, solid
+    # mesh 0: backdrop; mesh 1: two alpha geometries with different sources (mixed);
+    # mesh 2: one constant-alpha quad, instanced with and without an override
+    d.meshes.append(scenegen._mesh_from_parts([(*_quad(-8, 8, -4, 4, -1.0), 0)], [0]))
+    d.meshes.append(scenegen._mesh_from_parts([(*_quad(-4.5, -2.5, -1, 1, 0.0), 0),
+                                               (*_quad(-2.3, -0.3, -1, 1, 0.0), 1)], [1, 2]))
+    d.meshes.append(scenegen._mesh_from_parts([(*_quad(-1, 1, -1, 1, 0.0), 0)], [3]))
+    d.instances.append(scenegen.identity_instance(0))
+    d.instances.append(scenegen.identity_instance(1))
+    for x, ov in ((1.1, 1), (3.3, -1)):
+        m, n = scenegen.trs(translate=(x, 0.0, 0.0))
+        d.instances.append((m, n, 2, ov))
+    # a point light in front: the backdrop's shadow rays cross every quad
+    d.lights.append(dict(type=LIGHT_POINT, color=(1.0, 1.0, 1.0), intensity=20.0, radius=0.2, position=(0.0, 0.3, 2.5),
+                         forward=(0.0, 0.0, -1.0), inner_cone_deg=40.0, outer_cone_deg=50.0))
+    d.camera = scenegen.look_at_camera((0.0, 0.0, 6.0), (0.0, 0.0, 0.0), fov_y=40.0)
+    scene = runtime.Scene(gpu_ctx, d)
+    o = oracle_mod.OracleScene(d)
+    for frame in (0, 1):
+        g = scene.primary_hits(d.camera, d.width, d.height, frame)
+        r = o.primary_hits(frame=frame, brute=True)
+        assert same_hits(g, r)
+        inst, geom = g["instance"], g["geometry"]
+        assert np.any((inst == 1) & (geom == 0))             # textured geometry of the mixed mesh
+        assert not np.any((inst == 1) & (geom == 1))         # constant alpha 0.05: never accepted
+        assert np.any(inst == 2) and np.any(inst == 0)       # override texture: hits and see-through
+        assert np.any(inst == 3)                             # constant alpha 1.0: solid
+    img, _ = render_gpu(gpu_ctx, d, 8)
+    ref = o.render(frames=8)
+    assert relmse(img, ref) < 1e-4

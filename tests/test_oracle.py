@@ -276,6 +276,41 @@ def test_alpha_test_lets_rays_through_cutouts(oracle_mod):
     assert 0.25 < n_any / n_opq < 0.6   # ~40 % opaque leaf mask
 
 
+def _two_quads_one_mesh(second_override):
+    """One quad mesh (own material: blue) instanced twice; the first instance
+    overrides it to green, the second uses `second_override`."""
+    from helios_amd.abi import LIGHT_POINT
+    d = scenegen.SceneDesc(width=64, height=32, max_ray_bounces=1)
+    d.materials += [scenegen.material((0.9, 0.1, 0.1, 1.0)), scenegen.material((0.1, 0.9, 0.1, 1.0)),
+                    scenegen.material((0.1, 0.1, 0.9, 1.0))]
+    V = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float64)
+    d.meshes.append(scenegen._mesh_from_parts([(V, np.tile([0.0, 0.0, 1.0], (4, 1)), None,
+                                                np.array([0, 1, 2, 0, 2, 3], np.uint32), 0)], [2]))
+    for x, ov in ((-1.2, 1), (1.2, second_override)):
+        m, n = scenegen.trs(translate=(x, 0.0, 0.0))
+        d.instances.append((m, n, 0, ov))
+    d.lights.append(dict(type=LIGHT_POINT, color=(1.0, 1.0, 1.0), intensity=10.0, radius=0.1, position=(0.0, 0.0, 2.0),
+                         forward=(0.0, 0.0, -1.0), inner_cone_deg=40.0, outer_cone_deg=50.0))
+    d.camera = scenegen.look_at_camera((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), fov_y=35.0)
+    return d
+
+
+def test_unpacked_material_reads_as_material_zero(oracle_mod):
+    """scene.cpp:934,1143 (Q19): materials are packed only when a mesh is first
+    met, with that node's override; global_material_indices is an
+    unordered_map, so a later node of the same mesh whose own material (blue)
+    was never packed reads packed record 0 -- the first material packed,
+    here the first node's green override -- not its own blue and not the
+    scene's material 0 (red)."""
+    a = oracle_mod.OracleScene(_two_quads_one_mesh(-1)).render(frames=4)
+    b = oracle_mod.OracleScene(_two_quads_one_mesh(1)).render(frames=4)
+    assert np.array_equal(a, b)
+    for half in (a[:, 40:, :3], a[:, :24, :3]):
+        px = half.reshape(-1, 3)
+        lit = px[px.sum(axis=1) > 0.05]
+        assert len(lit) > 50 and np.all(lit[:, 1] > 10 * lit[:, 0]) and np.all(lit[:, 1] > 10 * lit[:, 2])
+
+
 # ---------------------------------------------------------------------------
 # Camera, sky, tonemap, accumulation, packing
 # ---------------------------------------------------------------------------
