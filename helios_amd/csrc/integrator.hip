@@ -759,25 +759,20 @@ __global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, con
 }
 
 // Megakernel variant: one thread runs a whole path (same device functions).
-// Whole paths per lane with path regeneration: a lane whose path has ended
-// takes the next path slot (claimed per wave like queue entries, from
-// kQShards contiguous slot ranges), so a wave is not held by its longest path.
-// One loop iteration traces one bounce (extension ray, shading, shadow ray).
-__global__ void k_mega_slots(uint32_t* counters, uint32_t total, uint32_t seg) {
-    const uint32_t sh = threadIdx.x;
-    if (sh < kQShards) {
-        const uint32_t first = sh * seg;
-        counters[ctr_index(kCtrExt0, sh)] = first < total ? min(seg, total - first) : 0u;
-        counters[ctr_index(kCtrExtHead, sh)] = 0u;
-    }
-}
-
+// The bounces after the first of every path, in one launch: each lane takes a
+// path from the depth-1 extension queue (ray, slot; throughput, radiance and
+// RNG from the path state) and traces it to its end, then takes the next
+// entry (claimed per wave like the traversal kernels do), so a wave is not
+// held by its longest path. One loop iteration is one bounce (extension ray,
+// shading, shadow ray).
 template <int W, int BS>
 __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
-                                                   uint32_t nv, const uint32_t* __restrict__ count, uint32_t* __restrict__ head,
-                                                   uint32_t seg, uint32_t refill, float4* __restrict__ Lout,
-                                                   unsigned long long* __restrict__ stats, uint32_t* __restrict__ spill,
-                                                   float4* __restrict__ feat, uint32_t feat_stride) {
+                                                   uint32_t nv, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                                                   const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count,
+                                                   uint32_t* __restrict__ head, uint32_t seg, uint32_t refill,
+                                                   const float4* __restrict__ Tin, const uint2* __restrict__ rngs,
+                                                   float4* __restrict__ Lout, unsigned long long* __restrict__ stats,
+                                                   uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     QueueCursor qc(count, head, seg, refill);
@@ -790,17 +785,18 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
     for (;;) {
         const uint32_t nq = qc.next(!active);
         if (nq != kNoEntry) {
-            slot = nq;
-            const SlotInfo si = slot_info(views, nv, slot);
-            view = si.view;
-            const ViewGPU& V = views[view];
-            rng = rng_init(si.x, si.y, V.first_frame + si.sample);
-            generate_ray(V, si.x, si.y, rng, o, d);
-            T = mk(1.f, 1.f, 1.f);
-            Lp = mk(0.f, 0.f, 0.f);
-            tmin = 0.001f;
-            rflags = 0;
-            depth = 0;
+            slot = q_slot[nq];
+            view = nv == 1 ? 0u : find_view(views, nv, slot);
+            const float4 o4 = q_o[nq];
+            o = ld3(o4);
+            tmin = o4.w;
+            d = ld3(q_d[nq]);
+            T = ld3(Tin[slot]);
+            Lp = ld3(Lout[slot]);
+            const uint2 rs = rngs[slot];
+            rng = RNG{rs.x, rs.y};
+            rflags = HH_TRACE_OPAQUE_BIT;
+            depth = 1;
             active = true;
         }
         if (__ballot(active) == 0ull) break;
@@ -811,17 +807,12 @@ __global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ 
             HitRec h;
             n_ext++;
             trace_ray<W, false>(S, o, d, tmin, 10000.0f, rflags, h, stk);
-            const bool features = depth == 0 && V.aux != nullptr;
             if (!h.found) {
-                f3 e = sample_env(S, d);
-                Lp = Lp + (depth == 0 ? e : T * e);
-                if (features) write_features(feat, feat_stride, slot, e, 0.0f, mk(0.f, 0.f, 0.f), true);
+                Lp = Lp + T * sample_env(S, d);
                 finish = true;
             } else {
                 Surface p;
                 populate(S, h.inst, h.tri, h.u, h.v, p);
-                if (features) write_features(feat, feat_stride, slot, p.albedo, h.t, p.normal, false);
-                if (depth == 0 && !is_black(p.emissive)) Lp = Lp + p.emissive;
                 ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, V.flags, rng);
                 if (sr.want) {
                     HitRec sh;
@@ -1206,51 +1197,59 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     // of extension per 1080p frame). Shadow and bounce rays diverge, and the
     // tuning's threshold is faster for them.
     const uint32_t refill_camera = 64u;
+    // The continuation megakernel takes new paths once a quarter of its lanes
+    // are idle (one 1080p frame: 3.21 ms at 32 lanes, 3.16 ms at 16).
+    const uint32_t refill_paths = 16u;
     uint32_t* ctr = wf.counters;
     const uint64_t seg64 = queue_segment(total, grid, BS);
-    if (!mega && (seg64 * kQShards > wf.qcap || seg64 * kQShards > 0xFFFFFFFFull)) return hipErrorInvalidValue;
+    if (seg64 * kQShards > wf.qcap || seg64 * kQShards > 0xFFFFFFFFull) return hipErrorInvalidValue;
     const uint32_t seg = (uint32_t)seg64;
-    if (mega) {
-        ProfScope ps(pf, PK_MEGAKERNEL, st);
-        const uint32_t mseg = (total + kQShards - 1) / kQShards;
-        k_mega_slots<<<1, 64, 0, st>>>(ctr, total, mseg);
-        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, ctr + ctr_index(kCtrExt0, 0),
-                                                 ctr + ctr_index(kCtrExtHead, 0), mseg, refill, wf.L, wf.stats, wf.spill,
-                                                 wf.feat, wf.capacity);
-    } else {
+    // one bounce of every path in the queue: extension rays, shading, shadow rays
+    auto bounce = [&](uint32_t d) {
+        const int a = d & 1, b = a ^ 1;
         {
-            ProfScope ps(pf, PK_GENERATE, st);
-            k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, ctr,
-                                                seg);
+            ProfScope ps(pf, PK_EXTEND, st);
+            k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
+                                                              ctr + ctr_index(kCtrExt0 + a, 0),
+                                                              ctr + ctr_index(kCtrExtHead, 0), seg,
+                                                              d == 0 ? refill_camera : refill,
+                                                              d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
+                                                              wf.hit_tri, wf.stats, tc, wf.spill);
         }
-        for (uint32_t d = 0; d < max_bounces; d++) {
-            const int a = d & 1, b = a ^ 1;
-            {
-                ProfScope ps(pf, PK_EXTEND, st);
-                k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
-                                                                  ctr + ctr_index(kCtrExt0 + a, 0),
-                                                                  ctr + ctr_index(kCtrExtHead, 0), seg,
-                                                                  d == 0 ? refill_camera : refill,
-                                                                  d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
-                                                                  wf.hit_tri, wf.stats, tc, wf.spill);
-            }
-            k_reset_queues<<<1, 64, 0, st>>>(ctr, kCtrExt0 + b);
-            {
-                ProfScope ps(pf, PK_SHADE, st);
-                k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
-                                                 ctr + ctr_index(kCtrExt0 + a, 0), seg,
-                                                 wf.hit, wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b],
-                                                 ctr + ctr_index(kCtrExt0 + b, 0), wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
-                                                 ctr + ctr_index(kCtrShadow, 0), wf.feat, wf.capacity);
-            }
-            {
-                ProfScope ps(pf, PK_SHADOW, st);
-                k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
-                                                                  ctr + ctr_index(kCtrShadow, 0),
-                                                                  ctr + ctr_index(kCtrShadowHead, 0), seg, refill, wf.L,
-                                                                  wf.stats, tc, wf.spill);
-            }
+        k_reset_queues<<<1, 64, 0, st>>>(ctr, kCtrExt0 + b);
+        {
+            ProfScope ps(pf, PK_SHADE, st);
+            k_shade<BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, d, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
+                                             ctr + ctr_index(kCtrExt0 + a, 0), seg,
+                                             wf.hit, wf.hit_tri, wf.T, wf.L, wf.rng, wf.q_o[b], wf.q_d[b], wf.q_slot[b],
+                                             ctr + ctr_index(kCtrExt0 + b, 0), wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+                                             ctr + ctr_index(kCtrShadow, 0), wf.feat, wf.capacity);
         }
+        {
+            ProfScope ps(pf, PK_SHADOW, st);
+            k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+                                                              ctr + ctr_index(kCtrShadow, 0),
+                                                              ctr + ctr_index(kCtrShadowHead, 0), seg, refill, wf.L,
+                                                              wf.stats, tc, wf.spill);
+        }
+    };
+    {
+        ProfScope ps(pf, PK_GENERATE, st);
+        k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, ctr, seg);
+    }
+    if (mega) {
+        // Camera rays as wavefront launches (one large coherent queue), then
+        // the remaining bounces of every path in one launch, so a frame is
+        // not held by the tails of seven more rounds of launches.
+        if (max_bounces > 0) bounce(0);
+        if (max_bounces > 1) {
+            ProfScope ps(pf, PK_MEGAKERNEL, st);
+            k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[1], wf.q_d[1], wf.q_slot[1],
+                                                     ctr + ctr_index(kCtrExt1, 0), ctr + ctr_index(kCtrExtHead, 0), seg,
+                                                     refill_paths, wf.T, wf.rng, wf.L, wf.stats, wf.spill);
+        }
+    } else {
+        for (uint32_t d = 0; d < max_bounces; d++) bounce(d);
     }
     uint32_t maxpix = 0;
     for (uint32_t i = 0; i < nv; i++) maxpix = std::max(maxpix, h_views[i].tile_w * h_views[i].tile_h);
