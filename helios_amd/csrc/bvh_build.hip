@@ -482,7 +482,9 @@ struct SahTree {
         parent[l] = uint32_t(id);
         parent[r] = uint32_t(id);
         const bool go_left = m - b > 1, go_right = e - m > 1;
-        if (go_left && go_right && depth < 5 && e - b > 65536) {
+        // the top five levels hand one side to a thread of its own (up to 32
+        // threads); the node ids do not depend on the schedule
+        if (go_left && go_right && depth < 5 && e - b > 4096) {
             std::thread t([&, b, m, lid, depth] { build(b, m, lid, depth + 1); });
             build(m, e, rid, depth + 1);
             t.join();
