@@ -540,6 +540,49 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_
     if (COUNT) flush_counts(tc, tcounts);
 }
 
+// Packets pay off once a wave holds samples of few pixels: measured per 1080p
+// frame of extension, 0.97 vs 0.99 ms at 4 samples per pixel in flight, 0.58 vs
+// 0.67 at 16; at one sample per pixel (64 pixels of a row) the single-frame
+// path was slower with them (3.46 vs 3.16 ms).
+constexpr uint32_t kPacketMinSpp = 4;
+
+// Camera rays in packets (trace_packet): a wave claims 64 consecutive entries
+// of the first extension queue, i.e. jittered samples of one pixel or
+// neighbouring pixels (pixel-major slots, 64-entry chunks), and traces them
+// together. Used when every view of the frame renders the same scene.
+template <int LY, int BS>
+__global__ void __launch_bounds__(BS) k_extend_packet(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
+                                                      const float4* __restrict__ q_o, const float4* __restrict__ q_d,
+                                                      const uint32_t* __restrict__ count, uint32_t* __restrict__ head,
+                                                      uint32_t seg, uint32_t rflags, float4* __restrict__ hit,
+                                                      uint32_t* __restrict__ hit_tri, unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t pstack[kPacketStack * (BS / 64)];
+    HH_LDS uint32_t* const stack = to_lds(&pstack[(threadIdx.x >> 6) * kPacketStack]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[0], (unsigned long long)queue_total(count));
+    const SceneGPU& S = scenes[views[0].scene];
+    QueueCursor qc(count, head, seg, 64u);
+    for (;;) {
+        const uint32_t q = qc.next(true);
+        const bool active = q != kNoEntry;
+        if (__ballot(active) == 0ull) break;
+        f3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 1.f);
+        float t0 = 0.0f, t1 = 0.0f;
+        if (active) {
+            const float4 a = q_o[q], b = q_d[q];
+            o = ld3(a);
+            t0 = a.w;
+            d = ld3(b);
+            t1 = b.w;
+        }
+        HitRec h;
+        trace_packet<LY>(S, active, o, d, t0, t1, rflags, h, stack);
+        if (active) {
+            hit[q] = make_float4(h.found ? h.t : -1.0f, h.u, h.v, __uint_as_float(h.inst));
+            hit_tri[q] = h.found ? h.tri : 0xFFFFFFFFu;
+        }
+    }
+}
+
 template <int BS>
 __global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                               uint32_t depth, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
@@ -1204,10 +1247,20 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     const uint64_t seg64 = queue_segment(total, grid, BS);
     if (seg64 * kQShards > wf.qcap || seg64 * kQShards > 0xFFFFFFFFull) return hipErrorInvalidValue;
     const uint32_t seg = (uint32_t)seg64;
+    // camera rays in packets when all views share one scene (the counting
+    // variant keeps the per-lane traversal, whose fetches it counts)
+    bool packets = !COUNT;
+    for (uint32_t i = 0; i < nv; i++)
+        packets = packets && h_views[i].scene == h_views[0].scene && h_views[i].spp >= kPacketMinSpp;
     // one bounce of every path in the queue: extension rays, shading, shadow rays
     auto bounce = [&](uint32_t d) {
         const int a = d & 1, b = a ^ 1;
-        {
+        if (d == 0 && packets) {
+            ProfScope ps(pf, PK_EXTEND, st);
+            k_extend_packet<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, wf.q_o[0], wf.q_d[0], ctr + ctr_index(kCtrExt0, 0),
+                                                        ctr + ctr_index(kCtrExtHead, 0), seg, 0u, wf.hit, wf.hit_tri,
+                                                        wf.stats);
+        } else {
             ProfScope ps(pf, PK_EXTEND, st);
             k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
                                                               ctr + ctr_index(kCtrExt0 + a, 0),

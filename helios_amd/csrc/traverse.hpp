@@ -435,4 +435,134 @@ __device__ inline bool trace_ray(const SceneGPU& S, f3 wo, f3 wd, float tmin, fl
     return h.found;
 }
 
+// ---------------------------------------------------------------------------
+// Packet traversal for neighbouring camera rays (the 64 lanes of a wave hold
+// jittered samples of one pixel, or neighbouring pixels): the wave walks one
+// shared tree path with a wave-uniform stack; a child is entered when any
+// active lane's ray enters it within that lane's [tmin, closest t so far],
+// and every lane keeps its own closest hit. Each lane's result is therefore
+// the closest (t, instance, geometry, primitive) over a superset of the
+// leaves its own traversal would reach, i.e. the same hit, computed with the
+// same node, instance and triangle arithmetic as Tracer. Children are ordered
+// by the entry distance of the first lane that enters them.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPacketStack = 128;  // wave-uniform stack entries (as deep as a lane stack)
+
+template <int L>
+__device__ inline void trace_packet(const SceneGPU& S, bool active, f3 wo, f3 wd, float tmin, float tmax, uint32_t rflags,
+                                    HitRec& h, HH_LDS uint32_t* stack) {
+    constexpr int W = layout_width(L);
+    const HH_GLOBAL void* nodes = (const HH_GLOBAL void*)S.nodes;
+    const HH_GLOBAL float4* tris = (const HH_GLOBAL float4*)S.tris;
+    const HH_GLOBAL InstTrav* insts = (const HH_GLOBAL InstTrav*)S.inst_trav;
+    const f3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+    f3 o = wo, inv = winv;
+    WT wt = wt_pre(wd);
+    const bool alpha_on = (rflags & 1u) == 0u;
+    h.found = false;
+    h.t = tmax;
+    h.u = h.v = 0.0f;
+    h.inst = 0xFFFFFFFFu;
+    h.tri = 0xFFFFFFFFu;
+    uint32_t hgeom = 0xFFFFFFFFu, hprim = 0xFFFFFFFFu;
+    const bool lane0 = __lane_id() == 0;
+    uint32_t cur = S.tlas_root, sp = 0, trips = 0;
+    int inst = -1;
+    auto push = [&](uint32_t r) {
+        if (sp < kPacketStack) {
+            if (lane0) stack[sp] = r;
+            sp++;
+        }
+    };
+    auto pop = [&]() { cur = sp == 0 ? kDone : (uint32_t)__builtin_amdgcn_readfirstlane((int)stack[--sp]); };
+    while (cur != kDone && trips < kMaxTrips) {
+        trips++;
+        if ((cur & kLeafBit) == 0u) {
+            float tn[W];
+            uint32_t ref[W];
+            node_test<L>(nodes, cur, o, inv, tmin, h.t, tn, ref);
+            // uniform sort keys: entry distance of the first entering lane
+            // (non-negative, so its bits order as unsigned), child index in
+            // the low bits; 0xFFFFFFFF = no lane enters
+            uint32_t key[W];
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const unsigned long long m = __ballot(active && tn[k] != INFINITY);
+                if (m == 0ull) {
+                    key[k] = 0xFFFFFFFFu;
+                } else {
+                    const int first = __ffsll((long long)m) - 1;
+                    const uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(tn[k]), first);
+                    key[k] = (bits & ~7u) | (uint32_t)k;
+                }
+            }
+#pragma unroll
+            for (int i = 1; i < W; i++)
+#pragma unroll
+                for (int j = W - 1; j >= i; j--)
+                    if (key[j] < key[j - 1]) {
+                        const uint32_t t = key[j];
+                        key[j] = key[j - 1];
+                        key[j - 1] = t;
+                    }
+            if (key[0] == 0xFFFFFFFFu) {
+                pop();
+                continue;
+            }
+            // child reference by (uniform) index, as a select chain
+            auto pick = [&](uint32_t i) {
+                uint32_t r = ref[0];
+#pragma unroll
+                for (int k = 1; k < W; k++) r = i == (uint32_t)k ? ref[k] : r;
+                return (uint32_t)__builtin_amdgcn_readfirstlane((int)r);
+            };
+#pragma unroll
+            for (int k = W - 1; k >= 1; k--)
+                if (key[k] != 0xFFFFFFFFu) push(pick(key[k] & 7u));
+            cur = pick(key[0] & 7u);
+        } else if (cur == kMarker) {
+            o = wo;
+            inv = winv;
+            wt = wt_pre(wd);
+            inst = -1;
+            pop();
+        } else if (inst < 0) {
+            inst = (int)leaf_start(cur);
+            const HH_GLOBAL InstTrav& it = insts[inst];
+            f3 od;
+            xform_ray(it.w2o, wo, wd, o, od);
+            inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+            wt = wt_pre(od);
+            push(kMarker);
+            cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)it.blas_root);
+        } else {
+            const uint32_t first = leaf_start(cur), ntri = leaf_count(cur);
+            if (active) {
+                for (uint32_t k = 0; k < ntri; k++) {
+                    const uint32_t ti = first + k;
+                    const float4 a = tris[3 * ti], b = tris[3 * ti + 1], c = tris[3 * ti + 2];
+                    float t, u, v;
+                    if (!tri_hit(wt, o, ld3(a), ld3(b), ld3(c), tmin, h.t, t, u, v)) continue;
+                    const uint32_t prim = __float_as_uint(a.w);
+                    const uint32_t gw = __float_as_uint(b.w);
+                    const uint32_t geom = gw & 0x7FFFFFFFu;
+                    if (h.found && t == h.t && !key_less((uint32_t)inst, geom, prim, h.inst, hgeom, hprim)) continue;
+                    if (alpha_on && (gw & 0x80000000u) == 0u) {
+                        if (hit_alpha(S, (uint32_t)inst, geom, ti, u, v) < 0.1f) continue;
+                    }
+                    h.found = true;
+                    h.t = t;
+                    h.u = u;
+                    h.v = v;
+                    h.inst = (uint32_t)inst;
+                    h.tri = ti;
+                    hgeom = geom;
+                    hprim = prim;
+                }
+            }
+            pop();
+        }
+    }
+}
+
 }  // namespace hh

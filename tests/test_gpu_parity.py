@@ -231,6 +231,25 @@ def test_bvh_layouts_bit_identical(gpu_ctx, small_field, field_pair, oracle_mod,
         gpu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("fif", [4, 8, 64])
+def test_camera_ray_packets_bit_identical(gpu_ctx, small_field, field_pair, fif):
+    """Camera rays are traced in wave-uniform packets (from 4 samples per
+    pixel in flight) except in the counting variant, which traces every ray on
+    its own: the images and the ray counts agree exactly (4 frames in flight:
+    packets of 16 neighbouring pixels; 64: the 64 samples of one pixel)."""
+    g, _ = field_pair
+    gpu_ctx.set_tuning(frames_in_flight=fif)
+    try:
+        packed, s0 = _render_gpu(gpu_ctx, g, small_field, frames=fif)
+        gpu_ctx.set_profiling(timing=True, counting=True)
+        lanes, s1 = _render_gpu(gpu_ctx, g, small_field, frames=fif)
+    finally:
+        gpu_ctx.set_profiling()
+        gpu_ctx.set_tuning()
+    assert np.array_equal(packed, lanes)
+    assert s0 == s1
+
+
 def test_frames_in_flight_bit_identical(gpu_ctx, small_field, field_pair):
     """Tracing several progressive frames per wavefront pass changes nothing."""
     g, _ = field_pair
