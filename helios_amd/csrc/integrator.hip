@@ -500,11 +500,11 @@ __device__ inline uint32_t queue_total(const uint32_t* count) {
     return n;
 }
 
-// Six waves per SIMD: the register budget (<= 80 VGPRs) that lets the LDS
-// stacks of 24 waves share one CU; measured faster than five waves with no
-// spills at all.
+// Five waves per SIMD (at most 102 VGPRs, no spills): measured 0.470 -> 0.452
+// ms of extension per 1080p frame against six waves with a few spills
+// (k_shadow, whose lanes finish at their first hit, is faster at six).
 template <int LY, int BS, int MODE, bool COUNT>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) k_extend(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                                const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                                const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count,
                                                uint32_t* __restrict__ head, uint32_t seg, uint32_t refill, uint32_t rflags,
@@ -583,8 +583,11 @@ __global__ void __launch_bounds__(BS) k_extend_packet(const SceneGPU* __restrict
     }
 }
 
+// Four waves per SIMD (at most 128 VGPRs, no spills): 0.229 -> 0.196 ms per
+// 1080p frame over the unconstrained three; five or six waves spill and are
+// slower.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_shade(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_shade(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views, uint32_t nv,
                                               uint32_t depth, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                               const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count,
                                               uint32_t seg, const float4* __restrict__ hit, const uint32_t* __restrict__ hit_tri,
@@ -801,15 +804,15 @@ __global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, con
     }
 }
 
-// Megakernel variant: one thread runs a whole path (same device functions).
 // The bounces after the first of every path, in one launch: each lane takes a
 // path from the depth-1 extension queue (ray, slot; throughput, radiance and
 // RNG from the path state) and traces it to its end, then takes the next
 // entry (claimed per wave like the traversal kernels do), so a wave is not
 // held by its longest path. One loop iteration is one bounce (extension ray,
-// shading, shadow ray).
+// shading, shadow ray). Four waves per SIMD (at most 128 VGPRs): one 1080p
+// frame 3.17 -> 3.03 ms over the unconstrained three.
 template <int W, int BS>
-__global__ void __launch_bounds__(BS) k_megakernel(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_megakernel(const SceneGPU* __restrict__ scenes, const ViewGPU* __restrict__ views,
                                                    uint32_t nv, const float4* __restrict__ q_o, const float4* __restrict__ q_d,
                                                    const uint32_t* __restrict__ q_slot, const uint32_t* __restrict__ count,
                                                    uint32_t* __restrict__ head, uint32_t seg, uint32_t refill,

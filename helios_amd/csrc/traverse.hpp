@@ -346,7 +346,7 @@ struct Tracer {
         }
         // (a branch-free variant that stores every child, sending unused
         // ones to a discard slot, measured slower: it raised register
-        // pressure past the six-wave budget)
+        // pressure past the kernels' wave budget)
 #pragma unroll
         for (int k = W - 1; k >= 1; k--)
             if (tn[k] != INFINITY) push(stk, ref[k]);
