@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--block", type=int, default=64)
     p.add_argument("--blocks-per-cu", type=int, default=32)
     p.add_argument("--frames-in-flight", type=int, default=128)
-    p.add_argument("--refill-lanes", type=int, default=32)
+    p.add_argument("--refill-lanes", type=int, default=24)
     p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
     p.add_argument("--bvh-builder", type=int, default=1, help="0 LBVH on the GPU, 1 binned SAH on the host")
     p.add_argument("--megakernel", action="store_true")

@@ -1668,10 +1668,10 @@ hh_status hh_context_autotune(hh_context ctx, const hh_view* views, uint32_t n, 
         const uint32_t fif[] = {1, 2, 4, 8, 16};
         const int blk[] = {64, 128, 256};
         const int bpc[] = {4, 8, 16, 32};
-        const uint32_t rf[] = {16, 32, 48, 64};
+        const uint32_t rf[] = {16, 24, 32, 48, 64};
         const int trav[] = {0, 1};
         for (int knob = 0; knob < 5 && st == HH_OK; knob++) {
-            const int count = knob == 0 ? 5 : knob == 1 ? 3 : knob == 2 ? 4 : knob == 3 ? 4 : 2;
+            const int count = knob == 0 ? 5 : knob == 1 ? 3 : knob == 2 ? 4 : knob == 3 ? 5 : 2;
             for (int c = 0; c < count && st == HH_OK; c++) {
                 Tuning t = best;
                 if (knob == 0) t.frames_in_flight = fif[c];

@@ -753,53 +753,80 @@ __global__ void k_reset_queues(uint32_t* counters, uint32_t next_ext) {
 // The feature planes follow the same recurrence, so they average the same
 // samples as the radiance (samples 1..f after frame f); M2 is Welford's sum
 // of squared luminance deviations about that running mean.
-__global__ void k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv, const float4* __restrict__ L,
-                             const float4* __restrict__ feat, uint32_t feat_stride) {
+// Samples of a pixel are consecutive slots (pixel-major), so a block stages
+// kAccChunk samples of its kAccBlock pixels in LDS with contiguous loads, then
+// each thread applies its pixel's samples in order from LDS.
+constexpr uint32_t kAccBlock = 256;
+constexpr uint32_t kAccChunk = 8;
+__global__ void __launch_bounds__(kAccBlock) k_accumulate(const ViewGPU* __restrict__ views, uint32_t nv,
+                                                          const float4* __restrict__ L, const float4* __restrict__ feat,
+                                                          uint32_t feat_stride) {
+    __shared__ float4 stage[kAccBlock * (kAccChunk + 1)];  // one padding entry per pixel row
     const ViewGPU& v = views[blockIdx.y];
-    const uint32_t npix = v.tile_w * v.tile_h;
+    const uint32_t npix = v.tile_w * v.tile_h, spp = v.spp;
     float4* const aux = v.aux;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
-        uint32_t y = v.tile_y + p / v.tile_w, x = v.tile_x + p % v.tile_w;
-        const size_t pi = (size_t)y * v.width + x;
-        float4* px = v.accum + pi;
-        float4 prev = *px;
-        float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
-        if (aux) {
-            a0 = aux[pi];
-            a1 = aux[(size_t)v.width * v.height + pi];
-        }
-        for (uint32_t s = 0; s < v.spp; s++) {
-            const uint32_t slot = v.slot_base + p * v.spp + s;
-            float4 l = L[slot];
-            float3 c;
-            if (v.flags & HH_FLAG_NO_CLAMP_BIT) c = make_float3(l.x, l.y, l.z);
-            else c = make_float3(fminf(l.x, 1.0f), fminf(l.y, 1.0f), fminf(l.z, 1.0f));
-            uint32_t f = v.first_frame + s;
-            const float lum_prev = luminance(make_float3(prev.x, prev.y, prev.z));
-            if (f == 0) prev = make_float4(c.x, c.y, c.z, 1.0f);
-            else {
-                float ff = (float)f;
-                prev = make_float4(prev.x + (c.x - prev.x) / ff, prev.y + (c.y - prev.y) / ff, prev.z + (c.z - prev.z) / ff, 1.0f);
-            }
+    // every loop bound below is uniform over the block, so all threads reach
+    // every barrier
+    for (uint32_t p0 = blockIdx.x * kAccBlock; p0 < npix; p0 += gridDim.x * kAccBlock) {
+        const uint32_t p = p0 + threadIdx.x;
+        const bool valid = p < npix;
+        const uint32_t npb = min(kAccBlock, npix - p0);
+        size_t pi = 0;
+        float4 prev = make_float4(0.f, 0.f, 0.f, 0.f), a0 = prev, a1 = prev;
+        if (valid) {
+            const uint32_t y = v.tile_y + p / v.tile_w, x = v.tile_x + p % v.tile_w;
+            pi = (size_t)y * v.width + x;
+            prev = v.accum[pi];
             if (aux) {
-                const float4 g0 = feat[slot], g1 = feat[feat_stride + slot];
-                const float xl = luminance(c);
-                if (f == 0) {
-                    a0 = g0;
-                    a1 = make_float4(g1.x, g1.y, g1.z, 0.0f);
-                } else {
-                    const float ff = (float)f;
-                    a0 = make_float4(a0.x + (g0.x - a0.x) / ff, a0.y + (g0.y - a0.y) / ff, a0.z + (g0.z - a0.z) / ff,
-                                     a0.w + (g0.w - a0.w) / ff);
-                    const float m2 = f == 1 ? 0.0f : a1.w + (xl - lum_prev) * (xl - luminance(make_float3(prev.x, prev.y, prev.z)));
-                    a1 = make_float4(a1.x + (g1.x - a1.x) / ff, a1.y + (g1.y - a1.y) / ff, a1.z + (g1.z - a1.z) / ff, m2);
+                a0 = aux[pi];
+                a1 = aux[(size_t)v.width * v.height + pi];
+            }
+        }
+        for (uint32_t s0 = 0; s0 < spp; s0 += kAccChunk) {
+            const uint32_t cn = min(kAccChunk, spp - s0);
+            __syncthreads();  // the previous round's LDS reads are done
+            for (uint32_t e = threadIdx.x; e < npb * cn; e += kAccBlock) {
+                const uint32_t pp = e / cn, ss = e - pp * cn;
+                stage[pp * (kAccChunk + 1) + ss] = L[v.slot_base + (p0 + pp) * spp + s0 + ss];
+            }
+            __syncthreads();
+            if (!valid) continue;
+            for (uint32_t k = 0; k < cn; k++) {
+                const uint32_t s = s0 + k;
+                const uint32_t slot = v.slot_base + p * spp + s;
+                const float4 l = stage[threadIdx.x * (kAccChunk + 1) + k];
+                float3 c;
+                if (v.flags & HH_FLAG_NO_CLAMP_BIT) c = make_float3(l.x, l.y, l.z);
+                else c = make_float3(fminf(l.x, 1.0f), fminf(l.y, 1.0f), fminf(l.z, 1.0f));
+                uint32_t f = v.first_frame + s;
+                const float lum_prev = luminance(make_float3(prev.x, prev.y, prev.z));
+                if (f == 0) prev = make_float4(c.x, c.y, c.z, 1.0f);
+                else {
+                    float ff = (float)f;
+                    prev = make_float4(prev.x + (c.x - prev.x) / ff, prev.y + (c.y - prev.y) / ff, prev.z + (c.z - prev.z) / ff, 1.0f);
+                }
+                if (aux) {
+                    const float4 g0 = feat[slot], g1 = feat[feat_stride + slot];
+                    const float xl = luminance(c);
+                    if (f == 0) {
+                        a0 = g0;
+                        a1 = make_float4(g1.x, g1.y, g1.z, 0.0f);
+                    } else {
+                        const float ff = (float)f;
+                        a0 = make_float4(a0.x + (g0.x - a0.x) / ff, a0.y + (g0.y - a0.y) / ff, a0.z + (g0.z - a0.z) / ff,
+                                         a0.w + (g0.w - a0.w) / ff);
+                        const float m2 = f == 1 ? 0.0f : a1.w + (xl - lum_prev) * (xl - luminance(make_float3(prev.x, prev.y, prev.z)));
+                        a1 = make_float4(a1.x + (g1.x - a1.x) / ff, a1.y + (g1.y - a1.y) / ff, a1.z + (g1.z - a1.z) / ff, m2);
+                    }
                 }
             }
         }
-        *px = prev;
-        if (aux) {
-            aux[pi] = a0;
-            aux[(size_t)v.width * v.height + pi] = a1;
+        if (valid) {
+            v.accum[pi] = prev;
+            if (aux) {
+                aux[pi] = a0;
+                aux[(size_t)v.width * v.height + pi] = a1;
+            }
         }
     }
 }
@@ -1309,10 +1336,10 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     }
     uint32_t maxpix = 0;
     for (uint32_t i = 0; i < nv; i++) maxpix = std::max(maxpix, h_views[i].tile_w * h_views[i].tile_h);
-    dim3 ag(std::max(1u, std::min(cdiv(maxpix, 256), 1024u)), nv);
+    dim3 ag(std::max(1u, std::min(cdiv(maxpix, kAccBlock), 65535u)), nv);
     {
         ProfScope ps(pf, PK_ACCUMULATE, st);
-        k_accumulate<<<ag, 256, 0, st>>>(d_views, nv, wf.L, wf.feat, wf.capacity);
+        k_accumulate<<<ag, kAccBlock, 0, st>>>(d_views, nv, wf.L, wf.feat, wf.capacity);
     }
     return hipGetLastError();
 }

@@ -64,7 +64,7 @@ def test_autotune_keeps_images_bit_identical(gpu_ctx):
     tuning, ms = runtime.autotune(gpu_ctx, tv, frames=2, bvh_width=True)
     assert ms > 0.0
     assert tuning["extend_block"] in (64, 128, 256) and tuning["frames_in_flight"] in (1, 2, 4, 8, 16)
-    assert tuning["bvh_width"] in (4, 8) and tuning["refill_lanes"] in (16, 32, 48, 64)
+    assert tuning["bvh_width"] in (4, 8) and tuning["refill_lanes"] in (16, 24, 32, 48, 64)
     after = [_single(gpu_ctx, v, spp) for v in views]
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
