@@ -394,8 +394,9 @@ __device__ inline float luminance(float3 c) { return 0.2126f * c.x + 0.7152f * c
 template <int BS>
 __global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ views, uint32_t nv, uint32_t total,
                                                  float4* __restrict__ q_o, float4* __restrict__ q_d,
-                                                 uint32_t* __restrict__ q_slot, float4* __restrict__ T, float4* __restrict__ L,
-                                                 uint2* __restrict__ rngs, uint32_t* __restrict__ counters, uint32_t seg) {
+                                                 uint32_t* __restrict__ q_slot, float4* __restrict__ L,
+                                                 uint2* __restrict__ rngs, uint32_t* __restrict__ counters, uint32_t seg,
+                                                 uint32_t clear_radiance) {
     // The first extension queue is dealt out to the segments in 64-entry
     // chunks: chunk c goes to segment c % kQShards at offset (c / kQShards) * 64.
     const uint32_t stride = gridDim.x * BS;
@@ -410,8 +411,9 @@ __global__ void __launch_bounds__(BS) k_generate(const ViewGPU* __restrict__ vie
         q_o[qi] = make_float4(o.x, o.y, o.z, 0.001f);
         q_d[qi] = make_float4(d.x, d.y, d.z, 10000.0f);
         q_slot[qi] = slot;
-        T[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-        L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // the first shade launch starts every path from T = 1, L = 0 without
+        // reading them; only a frame without bounces leaves L to this kernel
+        if (clear_radiance) L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         rngs[slot] = make_uint2(rng.s0, rng.s1);
     }
     if (blockIdx.x == 0 && threadIdx.x < kQShards) {
@@ -639,9 +641,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
             const SceneGPU& S = scenes[V.scene];
             float4 hh = hit[q];
             f3 dir = ld3(q_d[q]);
-            float4 T4 = T[slot];
+            float4 T4 = depth == 0 ? make_float4(1.f, 1.f, 1.f, 0.f) : T[slot];
             f3 Tp = ld3(T4);
-            float4 L4 = L[slot];
+            float4 L4 = depth == 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : L[slot];
             f3 Lp = ld3(L4);
             uint2 rs = rngs[slot];
             RNG rng{rs.x, rs.y};
@@ -1318,7 +1320,8 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     };
     {
         ProfScope ps(pf, PK_GENERATE, st);
-        k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.T, wf.L, wf.rng, ctr, seg);
+        k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.L, wf.rng, ctr, seg,
+                                            max_bounces == 0 ? 1u : 0u);
     }
     if (mega) {
         // Camera rays as wavefront launches (one large coherent queue), then
