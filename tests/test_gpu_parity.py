@@ -250,6 +250,25 @@ def test_camera_ray_packets_bit_identical(gpu_ctx, small_field, field_pair, fif)
     assert s0 == s1
 
 
+@pytest.mark.parametrize("tune", [dict(persistent_blocks_per_cu=1), dict(persistent_blocks_per_cu=1, extend_block=256),
+                                  dict(persistent_blocks_per_cu=3, extend_block=128, frames_in_flight=7)])
+def test_launch_shapes_bit_identical(gpu_ctx, small_field, field_pair, tune):
+    """The ray queues are split into segments sized from the launch shape
+    (a producer wave writes at most one extension and one shadow ray per entry
+    it reads); small persistent grids put many entries on each wave and fill
+    the segments to their bound. Every shape renders the same image."""
+    g, _ = field_pair
+    gpu_ctx.set_tuning(frames_in_flight=tune.get("frames_in_flight", 4))
+    ref, s0 = _render_gpu(gpu_ctx, g, small_field, frames=7)
+    gpu_ctx.set_tuning(**tune)
+    try:
+        img, s1 = _render_gpu(gpu_ctx, g, small_field, frames=7)
+    finally:
+        gpu_ctx.set_tuning()
+    assert np.array_equal(img, ref)
+    assert s0 == s1
+
+
 def test_frames_in_flight_bit_identical(gpu_ctx, small_field, field_pair):
     """Tracing several progressive frames per wavefront pass changes nothing."""
     g, _ = field_pair
