@@ -166,7 +166,6 @@ struct hh_integrator_t {
 };
 
 static uint64_t g_generation = 1;
-static uint64_t g_seq = 1;
 
 // ---------------------------------------------------------------------------
 // Host math (glm semantics evaluated in double)
