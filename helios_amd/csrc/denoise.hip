@@ -46,17 +46,19 @@ __device__ inline bf16x8 zero8() {
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution (forward, and the input gradient with flipped weights).
-// A workgroup of 4 waves computes 4 rows x TW pixels
+// A workgroup of kConvRows waves computes kConvRows rows x TW pixels
 // (TW = 16 NT) of all COUT channels. Input channels are taken 32 at a time:
-// the (4 + 2) x (TW + 2) halo tile of those channels is staged in LDS once
+// the (kConvRows + 2) x (TW + 2) halo tile of those channels is staged in LDS once
 // (coalesced 16-byte loads, zero outside the image and beyond CT; the
 // upsample + concat of the decoder is resolved here), then all 9 taps read
 // their B operands from LDS. The K order is (channel group, tap, channel):
 // the same products as the K order of the weights, summed in another order.
 // (Against one wave loading its B operands straight from global memory per
 // tap, staging the tile cut a 1080p forward pass from 1.45 to 1.21 ms.)
+constexpr int kConvRows = 4;              // output rows per workgroup, one wave each
+constexpr int kConvThreads = 64 * kConvRows;
 template <int C0, int UP0, int C1, int COUT, int EPI>
-__global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0, const __bf16* __restrict__ src1,
+__global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restrict__ src0, const __bf16* __restrict__ src1,
                                                      const __bf16* __restrict__ wgt, const float* __restrict__ bias,
                                                      const __bf16* __restrict__ mask, void* __restrict__ out, uint32_t N,
                                                      uint32_t H, uint32_t W) {
@@ -66,19 +68,19 @@ __global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0
     constexpr int MT = COUT / 16;
     constexpr int NT = COUT <= 96 ? 4 : 2;
     constexpr int TW = 16 * NT;
-    constexpr int TR = 4 + 2, TC = TW + 2;   // halo tile rows / columns
+    constexpr int TR = kConvRows + 2, TC = TW + 2;   // halo tile rows / columns
     constexpr int NG = (CT + 31) / 32;       // channel groups of 32
     static_assert(C0 % 8 == 0 && C1 % 8 == 0 && COUT % 16 == 0, "channel groups");
     constexpr int TILE = TR * TC * 4;
-    constexpr int SPT = (TILE + 255) / 256;  // staged 16-B chunks per thread
+    constexpr int SPT = (TILE + kConvThreads - 1) / kConvThreads;  // staged 16-B chunks per thread
     __shared__ bf16x8 tile[TILE];            // [row][col][4 chunks of 8 channels]
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t r = lane & 15u, g = lane >> 4;
-    const uint32_t segs = (W + TW - 1) / TW, rblocks = (H + 3) / 4;
+    const uint32_t segs = (W + TW - 1) / TW, rblocks = (H + kConvRows - 1) / kConvRows;
     const uint32_t blk = blockIdx.x;
     const uint32_t seg = blk % segs, rest = blk / segs;
     const uint32_t rb = rest % rblocks, n = rest / rblocks;
-    const int y0 = (int)(rb * 4), x0 = (int)(seg * TW);
+    const int y0 = (int)(rb * kConvRows), x0 = (int)(seg * TW);
     const uint32_t H0 = H / UP0, W0 = W / UP0;
     const int y = y0 + (int)wv;
 
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0
     auto load_group = [&](int cg) {
 #pragma unroll
         for (int i = 0; i < SPT; i++) {
-            const int e = (int)threadIdx.x + i * 256;
+            const int e = (int)threadIdx.x + i * kConvThreads;
             const int chunk = e & 3, pix = e >> 2;
             const int ty = pix / TC, tx = pix - ty * TC;
             const int sy = y0 - 1 + ty, sx = x0 - 1 + tx;
@@ -117,7 +119,7 @@ __global__ void __launch_bounds__(256) k_conv3x3(const __bf16* __restrict__ src0
         __syncthreads();  // every wave is done reading the previous group
 #pragma unroll
         for (int i = 0; i < SPT; i++) {
-            const int e = (int)threadIdx.x + i * 256;
+            const int e = (int)threadIdx.x + i * kConvThreads;
             if (e < TILE) tile[e] = stage[i];
         }
         __syncthreads();
@@ -597,8 +599,8 @@ template <int C0, int UP0, int C1, int COUT, int EPI>
 static void conv(hipStream_t st, const __bf16* s0, const __bf16* s1, const __bf16* w, const float* b, const __bf16* mask,
                  void* out, uint32_t N, uint32_t H, uint32_t W) {
     constexpr int NT = COUT <= 96 ? 4 : 2;
-    const size_t blocks = (size_t)N * ((H + 3) / 4) * ((W + 16 * NT - 1) / (16 * NT));
-    k_conv3x3<C0, UP0, C1, COUT, EPI><<<(uint32_t)blocks, 256, 0, st>>>(s0, s1, w, b, mask, out, N, H, W);
+    const size_t blocks = (size_t)N * ((H + kConvRows - 1) / kConvRows) * ((W + 16 * NT - 1) / (16 * NT));
+    k_conv3x3<C0, UP0, C1, COUT, EPI><<<(uint32_t)blocks, kConvThreads, 0, st>>>(s0, s1, w, b, mask, out, N, H, W);
 }
 
 template <int C0, int UP0, int C1, int COUT>
