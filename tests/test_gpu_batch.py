@@ -71,3 +71,32 @@ def test_autotune_keeps_images_bit_identical(gpu_ctx):
     gpu_ctx.set_tuning()
     for v in views:           # back to the default BVH width for later tests
         v["scene"].commit()
+
+
+def test_views_of_one_scene_in_packets(gpu_ctx):
+    """Views of one scene share the camera-ray packets of a pass, so a packet
+    can hold rays of two views (64-entry chunks straddle their slot ranges):
+    the batch renders the same images as the counting variant, which traces
+    every ray on its own lane."""
+    from helios_amd import runtime
+    d = scenegen.icosphere_field(seed=7, n_spheres=6, level=3, n_leaves=60, width=72, height=40, max_ray_bounces=6)
+    scene = runtime.Scene(gpu_ctx, d)
+    cams = [d.camera, scenegen.look_at_camera((10.0, 8.0, 30.0), (0.0, 2.0, 0.0), fov_y=50.0),
+            scenegen.look_at_camera((-20.0, 5.0, -10.0), (0.0, 1.0, 0.0), fov_y=60.0)]
+    sizes = [(72, 40), (50, 34), (33, 61)]
+    batch = [dict(scene=scene, surface=runtime.Surface(gpu_ctx, w, h), camera=c, spp=8, max_ray_bounces=6)
+             for c, (w, h) in zip(cams, sizes)]
+    gpu_ctx.set_tuning(frames_in_flight=8)
+    try:
+        runtime.render_batch(gpu_ctx, batch)
+        packed = [b["surface"].read() for b in batch]
+        gpu_ctx.set_profiling(timing=True, counting=True)
+        for b in batch:
+            b["surface"].clear()
+        runtime.render_batch(gpu_ctx, batch)
+        lanes = [b["surface"].read() for b in batch]
+    finally:
+        gpu_ctx.set_profiling()
+        gpu_ctx.set_tuning()
+    for a, b in zip(packed, lanes):
+        assert a[..., :3].any() and np.array_equal(a, b)
