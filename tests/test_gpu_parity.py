@@ -164,12 +164,16 @@ def test_radiance_field_matches_oracle(gpu_ctx, small_field, field_pair):
     assert relmse(img_g, img_o) < RELMSE_BAR
 
 
-def test_megakernel_equals_wavefront(gpu_ctx, small_field, field_pair):
+@pytest.mark.parametrize("bounces", [1, 2, 8])
+def test_megakernel_equals_wavefront(gpu_ctx, small_field, field_pair, bounces):
+    """The single-frame form (camera rays' bounce as wavefront launches, the
+    rest of every path in one megakernel launch; with one bounce no
+    megakernel runs) renders the wavefront's image bit for bit."""
     g, _ = field_pair
-    a, sa = _render_gpu(gpu_ctx, g, small_field, frames=3)
-    b, sb = _render_gpu(gpu_ctx, g, small_field, frames=3, flags=HH_FLAG_MEGAKERNEL)
-    assert relmse(a, b) < 1e-7
-    assert sa["extension_rays"] == sb["extension_rays"]
+    a, sa = _render_gpu(gpu_ctx, g, small_field, frames=3, bounces=bounces)
+    b, sb = _render_gpu(gpu_ctx, g, small_field, frames=3, flags=HH_FLAG_MEGAKERNEL, bounces=bounces)
+    assert np.array_equal(a, b)
+    assert sa["extension_rays"] == sb["extension_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
 
 
 def test_tonemap_matches_oracle(gpu_ctx, oracle_mod):
