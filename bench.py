@@ -7,7 +7,7 @@ displaced by 3-octave value noise (5 %), a 200x200 ground plane, 2,000
 alpha-tested leaf quads, GGX metallic-roughness materials, a Hosek-Wilkie sky
 (turbidity 3) with its sun at 30 degrees elevation plus 4 point lights;
 8 bounces, 1 sample per pixel per frame. A step is one progressive frame
-(2,073,600 paths). Frames are traced frames_in_flight at a time (default 128)
+(2,073,600 paths). Frames are traced frames_in_flight at a time (default 256)
 by one generate/extend/shade/shadow wavefront sequence replayed from a
 captured hipGraph; samples are still accumulated in frame order, so the
 image is bit-identical to tracing one frame per pass. Inputs are resident in
@@ -44,8 +44,8 @@ RAY_QUEUE_BYTES = 32 + 4 + 16 + 4   # ray in (o, d), slot, hit record out
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=256)
-    p.add_argument("--warmup", type=int, default=128)
+    p.add_argument("--steps", type=int, default=512)
+    p.add_argument("--warmup", type=int, default=256)
     p.add_argument("--width", type=int, default=1920)
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--bounces", type=int, default=8)
@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--bvh-compressed", type=int, default=0)
     p.add_argument("--block", type=int, default=64)
     p.add_argument("--blocks-per-cu", type=int, default=32)
-    p.add_argument("--frames-in-flight", type=int, default=128)
+    p.add_argument("--frames-in-flight", type=int, default=256)
     p.add_argument("--refill-lanes", type=int, default=24)
     p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
     p.add_argument("--bvh-builder", type=int, default=1, help="0 LBVH on the GPU, 1 binned SAH on the host")
