@@ -1443,10 +1443,15 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
         it->last_generation = scene->generation;
         it->last_scene = scene;
     }
+    // Frames per pass: up to frames_in_flight, and never more path slots than
+    // kMaxPathSlots (a 16384^2 surface still takes 8 frames per pass), so
+    // slot and queue indices stay within 32 bits.
+    const uint64_t px = (uint64_t)out->w * out->h;
+    const uint32_t fif = std::max<uint32_t>(1, std::min<uint64_t>(ctx->tune.frames_in_flight, kMaxPathSlots / px));
     // size the wavefront for the largest chunk first, so the statistics
     // counters are not reallocated after they are cleared
-    const uint32_t max_chunk = std::max<uint32_t>(1, std::min<uint32_t>(frames, ctx->tune.frames_in_flight));
-    HH_HIP(ensure_wavefront(ctx, out->w * out->h * max_chunk));
+    const uint32_t max_chunk = std::max<uint32_t>(1, std::min<uint32_t>(frames, fif));
+    HH_HIP(ensure_wavefront(ctx, (uint32_t)(px * max_chunk)));
     unsigned long long zero[2] = {0, 0};
     HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
     const bool mega = (it->s.flags & HH_FLAG_MEGAKERNEL) != 0;
@@ -1464,7 +1469,7 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
             f++;
             continue;
         }
-        uint32_t chunk = std::min(frames - f, std::max<uint32_t>(1, ctx->tune.frames_in_flight));
+        uint32_t chunk = std::min(frames - f, fif);
         chunk = std::min(chunk, it->s.max_samples - it->num_acc);
         ViewGPU v = make_view(*cam, out->w, out->h);
         auto t = it->tiles[it->tile_idx];
@@ -1568,7 +1573,9 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
     uint32_t chunk = std::max<uint32_t>(1, ctx->tune.frames_in_flight);
     if (ctx->tune.queue_capacity > 0)
         chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, (uint32_t)(ctx->tune.queue_capacity / std::max<size_t>(total_px, 1))));
-    HH_HIP(ensure_wavefront(ctx, (uint32_t)std::min<size_t>(total_px * std::min(chunk, max_spp), 0xFFFFFFFFull)));
+    chunk = std::max<uint32_t>(1, std::min<uint64_t>(chunk, kMaxPathSlots / std::max<size_t>(total_px, 1)));
+    HH_ARG(total_px * std::min(chunk, max_spp) <= kMaxPathSlots, "batch too large");
+    HH_HIP(ensure_wavefront(ctx, (uint32_t)(total_px * std::min(chunk, max_spp))));
     HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
     for (uint32_t s = 0; s < max_spp; s += chunk) {
         std::vector<ViewGPU> vg;
@@ -1591,7 +1598,7 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t n) {
             maxb = std::max(maxb, v.max_bounces);
             vg.push_back(v);
         }
-        HH_ARG(total <= 0xFFFFFFFFull, "batch too large");
+        HH_ARG(total <= kMaxPathSlots, "batch too large");
         std::string k = key;
         for (uint32_t i = 0; i < n; i++) k += "|" + std::to_string((uintptr_t)views[i].surface->d);
         hh_status st = run_frame(ctx, scenes, vg, (uint32_t)total, maxb, false, k);

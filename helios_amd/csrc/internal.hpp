@@ -53,6 +53,11 @@ __host__ __device__ inline uint32_t ctr_index(uint32_t kind, uint32_t shard) { r
 // and emits at most one extension and one shadow ray per queue entry it reads;
 // with grid-stride consumption every wave reads at most ceil(total / stride)
 // entries per lane. The initial queue is dealt out in 64-entry chunks.
+// Most path slots one wavefront pass may hold. Slot and queue indices are
+// 32-bit, and at ~216 B of path state per slot 2^30 slots take ~232 GB of the
+// 288 GB HBM; passes are shortened (fewer frames in flight) to stay below it.
+constexpr uint64_t kMaxPathSlots = 1ull << 30;
+
 inline uint64_t queue_segment(uint32_t total, uint32_t grid, uint32_t bs) {
     const uint64_t stride = (uint64_t)grid * bs, nwaves = stride / 64;
     const uint64_t per_wave = ((total + stride - 1) / stride) * 64;
