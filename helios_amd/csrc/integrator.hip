@@ -137,7 +137,62 @@ __device__ inline f3 sample_env_equirect(const SceneGPU& S, f3 d) {
               lerpf(lerpf(px(i0, j0, 2), px(i1, j0, 2), wx), lerpf(px(i0, j1, 2), px(i1, j1, 2), wx), wy));
 }
 
-// Cube-map sampling: Vulkan face selection, bilinear within the face.
+// Cube-map sampling: Vulkan face selection and seamless bilinear filtering
+// (the reference's LINEAR env sampler, vk.cpp:3557-3587). A bilinear tap that
+// falls off its face (i or j = -1 or n) reads the texel of the adjacent face
+// that holds that tap's centre: the centre (2i+1-n, 2j+1-n)/n is extended on
+// the face's plane and the face re-selected by its major axis, in exact
+// integer arithmetic. A tap off a corner is the mean of the three texels
+// that meet there. Texel centres lie half a texel from any texel border, so
+// integer floors and the float computation agree with the CPU restatement.
+__device__ inline int cube_select_i(int x, int y, int z, int& sc, int& tc, int& ma) {
+    const int ax = abs(x), ay = abs(y), az = abs(z);
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (x >= 0) { sc = -z; tc = -y; return 0; }
+        sc = z; tc = -y; return 1;
+    }
+    if (ay >= az) {
+        ma = ay;
+        if (y >= 0) { sc = x; tc = z; return 2; }
+        sc = x; tc = -z; return 3;
+    }
+    ma = az;
+    if (z >= 0) { sc = x; tc = -y; return 4; }
+    sc = -x; tc = -y; return 5;
+}
+// (face, i, j) with i or j one texel outside [0, n): the adjacent face's
+// texel index (face * n + j) * n + i.
+__device__ inline size_t cube_wrap(int face, int i, int j, int n) {
+    const int a = 2 * i + 1 - n, b = 2 * j + 1 - n;
+    int x, y, z;
+    switch (face) {  // inverse of the face selection: sc = a/n, tc = b/n
+        case 0: x = n; y = -b; z = -a; break;
+        case 1: x = -n; y = -b; z = a; break;
+        case 2: x = a; y = n; z = b; break;
+        case 3: x = a; y = -n; z = -b; break;
+        case 4: x = a; y = -b; z = n; break;
+        default: x = -a; y = -b; z = -n; break;
+    }
+    int sc, tc, ma;
+    const int f2 = cube_select_i(x, y, z, sc, tc, ma);
+    const int i2 = min(max((n * (sc + ma)) / (2 * ma), 0), n - 1);
+    const int j2 = min(max((n * (tc + ma)) / (2 * ma), 0), n - 1);
+    return ((size_t)f2 * n + j2) * n + i2;
+}
+__device__ inline float4 cube_texel(const float4* env, int n, int face, int i, int j) {
+    const bool oi = (unsigned)i >= (unsigned)n, oj = (unsigned)j >= (unsigned)n;
+    if (!oi && !oj) return env[((size_t)face * n + j) * n + i];
+    const int ci = min(max(i, 0), n - 1), cj = min(max(j, 0), n - 1);
+    if (oi && oj) {
+        const float4 c0 = env[((size_t)face * n + cj) * n + ci];
+        const float4 c1 = env[cube_wrap(face, i, cj, n)];
+        const float4 c2 = env[cube_wrap(face, ci, j, n)];
+        return make_float4(((c0.x + c1.x) + c2.x) / 3.0f, ((c0.y + c1.y) + c2.y) / 3.0f, ((c0.z + c1.z) + c2.z) / 3.0f,
+                           0.0f);
+    }
+    return env[cube_wrap(face, i, j, n)];
+}
 __device__ inline f3 sample_env(const SceneGPU& S, f3 d) {
     if (S.env_eq_w > 0u) return sample_env_equirect(S, d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
@@ -159,12 +214,21 @@ __device__ inline f3 sample_env(const SceneGPU& S, f3 d) {
     float x = s * (float)n - 0.5f, y = t * (float)n - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     float wx = x - fx, wy = y - fy;
-    int i0 = (int)fx, j0 = (int)fy;
-    int i1 = min(max(i0 + 1, 0), n - 1), j1 = min(max(j0 + 1, 0), n - 1);
-    i0 = min(max(i0, 0), n - 1);
-    j0 = min(max(j0, 0), n - 1);
-    const float4* base = S.env + (size_t)face * n * n;
-    float4 p00 = base[j0 * n + i0], p10 = base[j0 * n + i1], p01 = base[j1 * n + i0], p11 = base[j1 * n + i1];
+    const int i0 = (int)fx, j0 = (int)fy;
+    const float4* env = S.env;
+    float4 p00, p10, p01, p11;
+    if (i0 >= 0 && j0 >= 0 && i0 + 1 < n && j0 + 1 < n) {   // interior: one face
+        const float4* base = env + ((size_t)face * n + j0) * n + i0;
+        p00 = base[0];
+        p10 = base[1];
+        p01 = base[n];
+        p11 = base[n + 1];
+    } else {
+        p00 = cube_texel(env, n, face, i0, j0);
+        p10 = cube_texel(env, n, face, i0 + 1, j0);
+        p01 = cube_texel(env, n, face, i0, j0 + 1);
+        p11 = cube_texel(env, n, face, i0 + 1, j0 + 1);
+    }
     return mk(lerpf(lerpf(p00.x, p10.x, wx), lerpf(p01.x, p11.x, wx), wy),
               lerpf(lerpf(p00.y, p10.y, wx), lerpf(p01.y, p11.y, wx), wy),
               lerpf(lerpf(p00.z, p10.z, wx), lerpf(p01.z, p11.z, wx), wy));

@@ -88,6 +88,9 @@ def lib():
         L.orc_sample_cosine_lobe.argtypes = [vp, C.c_double, C.c_double, vp]
         L.orc_sample_ggx.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         L.orc_tonemap.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint32, vp]
+        L.orc_sample_env.argtypes = [vp, vp, C.c_uint32, C.c_int, vp]
+        L.orc_sample_light_probe.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, vp, vp]
+        L.orc_sample_light_probe.restype = C.c_int
         L.orc_tri_hit.argtypes = [vp, vp, vp, C.c_float, C.c_float, vp]
         L.orc_tri_hit.restype = C.c_int
         table = np.fromfile(HOSEK, dtype="<f8")
@@ -237,6 +240,23 @@ class OracleScene:
         n = lib().orc_ray_debug(self.h, C.byref(camera_struct(camera or d.camera)), C.byref(st), width, height,
                                 pixel[0], pixel[1], num_rays, frame, ptr(out), cap)
         return out[:min(n, cap)]
+
+    def sample_env(self, dirs, fp64=False):
+        """Environment radiance along each direction (n, 3)."""
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((len(dirs), 3), np.float32)
+        lib().orc_sample_env(self.h, ptr(dirs), len(dirs), int(fp64), ptr(out))
+        return out
+
+    def sample_light(self, light, pos, normal, rng_state, depth=0):
+        """One FP64 sample_light call on packed light `light`: returns
+        (Wi, Li, pdf, shadow tmax, draws) and advances rng_state (uint32[2])."""
+        p = np.ascontiguousarray(pos, np.float64)
+        n = np.ascontiguousarray(normal, np.float64)
+        out = np.zeros(9, np.float64)
+        if lib().orc_sample_light_probe(self.h, light, ptr(p), ptr(n), depth, ptr(rng_state), ptr(out)):
+            raise ValueError("bad light index or scene not committed")
+        return out[0:3].copy(), out[3:6].copy(), float(out[6]), float(out[7]), int(out[8])
 
     def trace_rays(self, rays, flags=0, brute=False, fp64=False):
         rays = np.ascontiguousarray(rays)

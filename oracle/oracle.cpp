@@ -214,8 +214,79 @@ static void sample_bilinear_repeat(const Texture& t, R s, R u_t, R out[4]) {
     }
 }
 
-// Cube sampling (Vulkan face selection), bilinear within the face with
-// clamp-to-edge. Face texels are row-major, t = 0 at row 0.
+// Cube sampling with Vulkan's face selection and seamless bilinear filtering
+// (the reference's env sampler is LINEAR, vk.cpp:3557-3587; Vulkan cube
+// lookups filter across face edges, "Cube Map Edge Handling"). Face texels
+// are row-major, t = 0 at row 0.
+//
+// A bilinear tap (i, j) outside a face (i or j = -1 or n) is the texel of the
+// adjacent face that holds that texel's centre: the centre is extended on the
+// face's plane, (2i+1-n, 2j+1-n) over n, and the face is re-selected by the
+// major axis of that point (all in integers, exact). A tap outside on both axes
+// (a corner) is the mean of the three texels that meet there: this face's
+// corner texel and its two neighbours across the edges (Vulkan leaves the
+// corner implementation-defined).
+static void cube_face_point(int face, int64_t a, int64_t b, int64_t m, int64_t P[3]) {
+    switch (face) {  // inverse of the face selection below: sc = a/m, tc = b/m
+        case 0: P[0] = m; P[1] = -b; P[2] = -a; break;
+        case 1: P[0] = -m; P[1] = -b; P[2] = a; break;
+        case 2: P[0] = a; P[1] = m; P[2] = b; break;
+        case 3: P[0] = a; P[1] = -m; P[2] = -b; break;
+        case 4: P[0] = a; P[1] = -b; P[2] = m; break;
+        default: P[0] = -a; P[1] = -b; P[2] = -m; break;
+    }
+}
+static int cube_select(int64_t x, int64_t y, int64_t z, int64_t& sc, int64_t& tc, int64_t& ma) {
+    const int64_t ax = x < 0 ? -x : x, ay = y < 0 ? -y : y, az = z < 0 ? -z : z;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (x >= 0) { sc = -z; tc = -y; return 0; }
+        sc = z; tc = -y; return 1;
+    }
+    if (ay >= az) {
+        ma = ay;
+        if (y >= 0) { sc = x; tc = z; return 2; }
+        sc = x; tc = -z; return 3;
+    }
+    ma = az;
+    if (z >= 0) { sc = x; tc = -y; return 4; }
+    sc = -x; tc = -y; return 5;
+}
+// (face, i, j) with one of i, j just outside [0, n): the adjacent face's texel.
+static void cube_wrap(int face, int64_t i, int64_t j, int64_t n, int& f2, int64_t& i2, int64_t& j2) {
+    int64_t P[3], sc, tc, ma;
+    cube_face_point(face, 2 * i + 1 - n, 2 * j + 1 - n, n, P);
+    f2 = cube_select(P[0], P[1], P[2], sc, tc, ma);
+    i2 = (n * (sc + ma)) / (2 * ma);
+    j2 = (n * (tc + ma)) / (2 * ma);
+    i2 = i2 < 0 ? 0 : (i2 >= n ? n - 1 : i2);
+    j2 = j2 < 0 ? 0 : (j2 >= n ? n - 1 : j2);
+}
+template <typename R>
+static void cube_texel(const std::vector<float>& faces, uint32_t size, int face, int64_t i, int64_t j, R out[3]) {
+    const int64_t n = size;
+    auto at = [&](int f, int64_t ii, int64_t jj, int c) {
+        return R(faces[((size_t(f) * size + size_t(jj)) * size + size_t(ii)) * 4 + c]);
+    };
+    const bool oi = i < 0 || i >= n, oj = j < 0 || j >= n;
+    if (!oi && !oj) {
+        for (int c = 0; c < 3; c++) out[c] = at(face, i, j, c);
+        return;
+    }
+    const int64_t ci = i < 0 ? 0 : (i >= n ? n - 1 : i), cj = j < 0 ? 0 : (j >= n ? n - 1 : j);
+    int f1;
+    int64_t i1, j1;
+    if (oi && oj) {
+        int f2;
+        int64_t i2, j2;
+        cube_wrap(face, i, cj, n, f1, i1, j1);
+        cube_wrap(face, ci, j, n, f2, i2, j2);
+        for (int c = 0; c < 3; c++) out[c] = ((at(face, ci, cj, c) + at(f1, i1, j1, c)) + at(f2, i2, j2, c)) / R(3);
+        return;
+    }
+    cube_wrap(face, i, j, n, f1, i1, j1);
+    for (int c = 0; c < 3; c++) out[c] = at(f1, i1, j1, c);
+}
 template <typename R>
 static V3<R> sample_cube(const std::vector<float>& faces, uint32_t size, V3<R> d) {
     R ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
@@ -236,15 +307,16 @@ static V3<R> sample_cube(const std::vector<float>& faces, uint32_t size, V3<R> d
     R x = s * R(size) - R(0.5), y = t * R(size) - R(0.5);
     R fx = std::floor(x), fy = std::floor(y);
     R wx = x - fx, wy = y - fy;
-    int64_t n = size;
-    auto cl = [n](int64_t i) { return i < 0 ? 0 : (i >= n ? n - 1 : i); };
-    int64_t i0 = cl((int64_t)fx), i1 = cl((int64_t)fx + 1), j0 = cl((int64_t)fy), j1 = cl((int64_t)fy + 1);
-    const float* base = &faces[size_t(face) * size * size * 4];
-    auto px = [&](int64_t i, int64_t j, int c) { return R(base[(size_t(j) * size + i) * 4 + c]); };
+    const int64_t i0 = (int64_t)fx, j0 = (int64_t)fy;
+    R p00[3], p10[3], p01[3], p11[3];
+    cube_texel<R>(faces, size, face, i0, j0, p00);
+    cube_texel<R>(faces, size, face, i0 + 1, j0, p10);
+    cube_texel<R>(faces, size, face, i0, j0 + 1, p01);
+    cube_texel<R>(faces, size, face, i0 + 1, j0 + 1, p11);
     R out[3];
     for (int c = 0; c < 3; c++) {
-        R top = lerp_(px(i0, j0, c), px(i1, j0, c), wx);
-        R bot = lerp_(px(i0, j1, c), px(i1, j1, c), wx);
+        R top = lerp_(p00[c], p10[c], wx);
+        R bot = lerp_(p01[c], p11[c], wx);
         out[c] = lerp_(top, bot, wy);
     }
     return V3<R>(out[0], out[1], out[2]);
@@ -1123,6 +1195,7 @@ struct Tracer {
     bool brute;
     Stats st;
     uint32_t rr_start = 0;  // settings.rr_start_depth (0 = the reference: RR at every bounce)
+    R last_tmax = R(0);     // tmax of the last shadow ray (probe only)
 
     V3<R> env(V3<R> d) const {
         if (s.env_eq_w > 0) return sample_equirect<R>(s.env_eq, s.env_eq_w, d);
@@ -1204,6 +1277,7 @@ struct Tracer {
             pdf = dsq / gmax(kEps<R>(), ct * area);
         }
         st.shadow++;
+        last_tmax = tmax;
         Hit<R> h = trace<R>(s, origin, Wi, tmin, tmax, ray_flags, false, brute);
         return h.found ? Li * R(0) : Li;
     }
@@ -1723,6 +1797,50 @@ int orc_trace_rays(void* sp, const hh_ray* rays, uint32_t n, uint32_t flags, int
             o.instance = o.geometry = o.primitive = 0xFFFFFFFFu;
         }
     }
+    return 0;
+}
+// Environment lookup (cube or equirect, as the miss shader and the env light
+// see it) for n directions.
+void orc_sample_env(void* sp, const float* dirs, uint32_t n, int fp64, float* out) {
+    Scene& s = *(Scene*)sp;
+    for (uint32_t i = 0; i < n; i++) {
+        if (fp64) {
+            Tracer<double> tr{s, 0.0, s.num_lights, 1, 0, false};
+            V3<double> e = tr.env(V3<double>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+            out[3 * i] = (float)e.x; out[3 * i + 1] = (float)e.y; out[3 * i + 2] = (float)e.z;
+        } else {
+            Tracer<float> tr{s, 0.0f, s.num_lights, 1, 0, false};
+            V3<float> e = tr.env(V3<float>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+            out[3 * i] = e.x; out[3 * i + 1] = e.y; out[3 * i + 2] = e.z;
+        }
+    }
+}
+// One sample_light call (FP64) on packed light record `light` from a surface
+// point with geometric and shading normal `nrm`: out = Wi xyz, Li xyz (times
+// visibility), pdf, shadow-ray tmax, RNG draws consumed. The RNG state is
+// advanced in place.
+int orc_sample_light_probe(void* sp, uint32_t light, const double* pos, const double* nrm, uint32_t depth,
+                           uint32_t st[2], double* out) {
+    Scene& s = *(Scene*)sp;
+    if (!s.committed || light >= s.light_data.size()) return 1;
+    Tracer<double> tr{s, 0.0, s.num_lights, 1, 0, false};
+    Surface<double> p{};
+    p.vertex.pos = V3<double>(pos[0], pos[1], pos[2]);
+    p.normal = p.vertex.normal = V3<double>(nrm[0], nrm[1], nrm[2]);
+    RNG r{st[0], st[1]};
+    const RNG r0 = r;
+    V3<double> Wi(0, 0, 0);
+    double pdf = 0;
+    V3<double> Li = tr.sample_light(p, s.light_data[light], depth, r, Wi, pdf);
+    out[0] = Wi.x; out[1] = Wi.y; out[2] = Wi.z;
+    out[3] = Li.x; out[4] = Li.y; out[5] = Li.z;
+    out[6] = pdf;
+    out[7] = tr.last_tmax;
+    uint32_t draws = 0;
+    for (RNG q = r0; (q.s0 != r.s0 || q.s1 != r.s1) && draws < 16; draws++) rng_next(q);
+    out[8] = draws;
+    st[0] = r.s0;
+    st[1] = r.s1;
     return 0;
 }
 // Unit-level entry points for known-answer tests.
