@@ -1086,6 +1086,10 @@ hh_status hh_scene_commit(hh_scene s) {
         HH_HIP(upload(s, emit_cdf, (float**)&g.emit_cdf));
     }
     g.bvh_width = (uint32_t)layout;
+    {
+        const std::vector<unsigned long long> zeros(kNumFaults, 0ull);
+        HH_HIP(upload(s, zeros, &g.faults));
+    }
     HH_HIP(upload(s, trav, (InstTrav**)&g.inst_trav));
     HH_HIP(upload(s, inst_data, (InstanceData**)&g.inst));
     HH_HIP(upload(s, mat_data, (MaterialData**)&g.mats));
@@ -1103,6 +1107,25 @@ hh_status hh_scene_commit(hh_scene s) {
     s->committed = true;
     s->dirty = false;
     s->generation = g_generation++;
+    return HH_OK;
+}
+
+hh_status hh_scene_traversal_faults(hh_scene s, uint64_t out[3], int reset) {
+    HH_ARG(s && out, "null argument");
+    if (!s->committed || !s->gpu.faults) {
+        out[0] = out[1] = out[2] = 0;
+        return HH_OK;
+    }
+    HH_HIP(hipSetDevice(s->ctx->device));
+    unsigned long long v[kNumFaults];
+    HH_HIP(hipMemcpyAsync(v, s->gpu.faults, sizeof(v), hipMemcpyDeviceToHost, s->ctx->stream));
+    HH_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (int k = 0; k < kNumFaults; k++) out[k] = v[k];
+    if (reset) {
+        const unsigned long long z[kNumFaults] = {0, 0, 0};
+        HH_HIP(hipMemcpyAsync(s->gpu.faults, z, sizeof(z), hipMemcpyHostToDevice, s->ctx->stream));
+        HH_HIP(hipStreamSynchronize(s->ctx->stream));
+    }
     return HH_OK;
 }
 

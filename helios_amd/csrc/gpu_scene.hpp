@@ -147,7 +147,16 @@ struct SceneGPU {
     const float4* emit_tri;       // 3 per triangle: world v0 | area, v1, v2
     const float4* emit_le;        // emission rgb | selection probability
     const float* emit_cdf;        // inclusive cumulative probability
+    // Traversal limit counters (kFault*), incremented only when a ray's
+    // traversal was cut short by a stack bound or the trip bound.
+    unsigned long long* faults;
 };
+
+// Indices into SceneGPU::faults.
+constexpr int kFaultLaneStack = 0;    // a per-lane stack push dropped (LDS + spill full)
+constexpr int kFaultPacketStack = 1;  // a wave-uniform packet stack push dropped
+constexpr int kFaultTrips = 2;        // a ray stopped at the node/leaf visit bound
+constexpr int kNumFaults = 3;
 
 // Per-view launch constants (the 192-B PathTraceConsts block, reduced to the
 // fields the path reads) plus the batching bookkeeping.

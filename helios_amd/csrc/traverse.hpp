@@ -246,6 +246,7 @@ __device__ inline bool key_less(uint32_t i1, uint32_t g1, uint32_t p1, uint32_t 
 // ---------------------------------------------------------------------------
 constexpr uint32_t kDone = 0xFFFFFFFEu;     // never a valid child reference
 constexpr uint32_t kMaxTrips = 1u << 22;    // hard bound on node + leaf visits per ray
+constexpr uint32_t kStackFull = 0xFFFFFFFFu;  // trip count of a ray whose stack overflowed
 
 // Step shapes: if-if visits one node or leaf per step; while-while visits
 // inner nodes until the lane reaches a leaf (or finishes), then one leaf.
@@ -298,12 +299,17 @@ struct Tracer {
         cur = scene.tlas_root;
     }
 
-    // Entries beyond the stack capacity are dropped (never written out of
-    // bounds); the capacity is far above the depth of the built trees.
+    // A push beyond the stack capacity (never written out of bounds) ends
+    // the ray: `trips` takes kStackFull, which stops the visit loop, and the
+    // scene's fault counter records it; the capacity is far above the depth
+    // of the built trees.
     __device__ inline void push(const LaneStack& stk, uint32_t r) {
         if (sp < (uint32_t)kLdsStack) stk.lds[sp * stk.lds_stride] = r;
         else if (sp < (uint32_t)(kLdsStack + kSpillStack)) stk.spill[(sp - kLdsStack) * stk.spill_stride] = r;
-        else return;
+        else {
+            trips = kStackFull;
+            return;
+        }
         sp++;
     }
     __device__ inline void pop(const LaneStack& stk) {
@@ -397,7 +403,11 @@ struct Tracer {
             visit_inner(stk);
             return false;
         }
-        if (cur == kDone || trips >= kMaxTrips) return true;
+        if (cur == kDone) return true;
+        if (trips >= kMaxTrips) {  // visit bound reached, or a full stack
+            atomicAdd(&S->faults[trips == kStackFull ? kFaultLaneStack : kFaultTrips], 1ull);
+            return true;
+        }
         trips++;
         if (cur == kMarker) {
             o = wo;
@@ -472,6 +482,8 @@ __device__ inline void trace_packet(const SceneGPU& S, bool active, f3 wo, f3 wd
         if (sp < kPacketStack) {
             if (lane0) stack[sp] = r;
             sp++;
+        } else {
+            trips = kStackFull;  // ends the packet; counted below
         }
     };
     auto pop = [&]() { cur = sp == 0 ? kDone : (uint32_t)__builtin_amdgcn_readfirstlane((int)stack[--sp]); };
@@ -563,6 +575,7 @@ __device__ inline void trace_packet(const SceneGPU& S, bool active, f3 wo, f3 wd
             pop();
         }
     }
+    if (cur != kDone && lane0) atomicAdd(&S.faults[trips == kStackFull ? kFaultPacketStack : kFaultTrips], 1ull);
 }
 
 }  // namespace hh

@@ -57,6 +57,7 @@ _SIGS = {
     "hh_scene_sky_coefficients": [C.c_void_p, C.c_void_p],
     "hh_scene_read_env_equirect": [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)],
     "hh_scene_bvh_stats": [C.c_void_p, C.c_void_p],
+    "hh_scene_traversal_faults": [C.c_void_p, C.c_void_p, C.c_int],
     "hh_surface_create": [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)],
     "hh_surface_destroy": [C.c_void_p],
     "hh_surface_clear": [C.c_void_p],
@@ -292,6 +293,13 @@ class Scene:
         out = np.zeros(4, np.uint64)
         check(lib().hh_scene_bvh_stats(self.h, ptr(out)))
         return dict(blas_nodes=int(out[0]), triangles=int(out[1]), tlas_nodes=int(out[2]), meshes=int(out[3]))
+
+    def traversal_faults(self, reset=False):
+        """Rays cut short by a traversal bound since commit (or the last reset):
+        dropped lane-stack pushes, dropped packet-stack pushes, visit-bound stops."""
+        out = np.zeros(3, np.uint64)
+        check(lib().hh_scene_traversal_faults(self.h, ptr(out), int(reset)))
+        return dict(lane_stack=int(out[0]), packet_stack=int(out[1]), trips=int(out[2]))
 
     def trace_rays(self, rays, flags=0):
         rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)

@@ -413,6 +413,95 @@ def icosphere_field(seed=0, n_spheres=64, level=5, n_leaves=2000, width=1920, he
     return d
 
 
+def textured_showcase(width=96, height=64, max_ray_bounces=4, seed=5):
+    """Every texture branch of populate_surface_properties
+    (path_trace_rchit.glsl:190-252) and a spot light (:319-341):
+
+    * ground: sRGB albedo checker, tangent-space normal map (bumps), roughness
+      from the G channel and metallic from the B channel of two separate
+      textures (glTF's channel convention), UVs repeating four times;
+    * sphere: constant albedo, emission from an RGBA32F texture (bands up to 3);
+    * box: constant rough metal;
+    * lights: a spot (inner 14, outer 32 degrees) onto the ground, a point light
+      and the sun (so the Hosek-Wilkie sky and the env light are present).
+    """
+    from .abi import TEXTURE_RGBA8_UNORM, TEXTURE_RGBA32F
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 64
+    y, x = np.mgrid[0:n, 0:n].astype(np.float64)
+    u, v = (x + 0.5) / n, (y + 0.5) / n
+    d = SceneDesc()
+    checker = ((np.floor(u * 8) + np.floor(v * 8)) % 2).astype(bool)
+    alb = np.zeros((n, n, 4), np.uint8)
+    alb[..., 0] = np.where(checker, 210, 60)
+    alb[..., 1] = np.where(checker, 180, 90)
+    alb[..., 2] = np.where(checker, 120, 160)
+    alb[..., 3] = 255
+    d.textures.append((n, n, TEXTURE_RGBA8_SRGB, alb))
+    # bumps: height h = sin(2 pi 3u) sin(2 pi 3v) * 0.15 -> tangent normal (-dh/du, -dh/dv, 1)
+    k = 2 * math.pi * 3
+    dhu = 0.15 * k * np.cos(k * u) * np.sin(k * v) / 6.0
+    dhv = 0.15 * k * np.sin(k * u) * np.cos(k * v) / 6.0
+    tn = np.stack([-dhu, -dhv, np.ones_like(u)], -1)
+    tn /= np.linalg.norm(tn, axis=-1, keepdims=True)
+    nm = np.zeros((n, n, 4), np.uint8)
+    nm[..., :3] = np.clip(np.round((tn * 0.5 + 0.5) * 255), 0, 255).astype(np.uint8)
+    nm[..., 3] = 255
+    d.textures.append((n, n, TEXTURE_RGBA8_UNORM, nm))
+    rough = np.zeros((n, n, 4), np.uint8)
+    rough[..., 0] = rng.integers(0, 256, (n, n))                      # not read (channel 1 is)
+    rough[..., 1] = np.clip(np.round((0.15 + 0.8 * (0.5 + 0.5 * np.sin(2 * math.pi * 2 * u))) * 255), 0, 255)
+    rough[..., 2] = rng.integers(0, 256, (n, n))
+    rough[..., 3] = 255
+    d.textures.append((n, n, TEXTURE_RGBA8_UNORM, rough))
+    metal = np.zeros((n, n, 4), np.uint8)
+    metal[..., 1] = rng.integers(0, 256, (n, n))
+    metal[..., 2] = np.where((np.floor(v * 4) % 2) == 0, 255, 0)   # metallic stripes in B
+    metal[..., 3] = 255
+    d.textures.append((n, n, TEXTURE_RGBA8_UNORM, metal))
+    em = np.zeros((n, n, 4), np.float32)
+    em[..., 0] = 3.0 * (np.floor(v * 6) % 2)
+    em[..., 1] = 1.5 * (np.floor(u * 6) % 2)
+    em[..., 2] = 0.5
+    em[..., 3] = 1.0
+    d.textures.append((n, n, TEXTURE_RGBA32F, em))
+    d.materials.append(material(albedo_texture=0, normal_texture=1, roughness_texture=2, roughness_channel=1,
+                                metallic_texture=3, metallic_channel=2))
+    d.materials.append(material(albedo=linear_albedo((0.3, 0.3, 0.35)), emissive_texture=4, roughness=0.6))
+    d.materials.append(material(albedo=linear_albedo((0.9, 0.6, 0.3)), metallic=1.0, roughness=0.3))
+    # ground with UVs 0..4 (REPEAT)
+    pos, nrm, _, idx = _quad((-6, 0, -6), (6, 0, -6), (6, 0, 6), (-6, 0, 6), (0, 1, 0))
+    uv = np.array([[0, 0], [4, 0], [4, 4], [0, 4]], np.float64)
+    d.meshes.append(_mesh_from_parts([(pos, nrm, uv, idx, 0)], [0]))
+    V, F = icosphere(3)
+    uvs = np.stack([0.5 + np.arctan2(V[:, 2], V[:, 0]) / (2 * np.pi), 0.5 - np.arcsin(np.clip(V[:, 1], -1, 1)) / np.pi], 1)
+    d.meshes.append(_mesh_from_parts([(V * 0.8, V, uvs, F.ravel(), 0)], [1]))
+    bp, bn, bu, bi = _box((-0.5, 0.0, -0.5), (0.5, 1.2, 0.5))
+    d.meshes.append(_mesh_from_parts([(bp, bn, bu, bi, 0)], [2]))
+    d.instances.append(identity_instance(0))
+    m, nmx = trs(translate=(-1.2, 0.8, 0.3))
+    d.instances.append((m, nmx, 1, -1))
+    c, s_ = math.cos(0.5), math.sin(0.5)
+    m, nmx = trs(translate=(1.4, 0.0, -0.4), rot=np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]]))
+    d.instances.append((m, nmx, 2, -1))
+    spot_pos = np.array([0.3, 3.5, 1.2])
+    fwd = np.array([0.0, 0.0, 0.2]) - spot_pos
+    fwd /= np.linalg.norm(fwd)
+    d.lights.append(dict(type=LIGHT_SPOT, color=(1.0, 0.9, 0.7), intensity=18.0, radius=0.2,
+                         position=tuple(spot_pos), forward=tuple(fwd), inner_cone_deg=14.0, outer_cone_deg=32.0))
+    d.lights.append(dict(type=LIGHT_POINT, color=(0.6, 0.7, 1.0), intensity=6.0, radius=0.3,
+                         position=(-2.5, 2.5, 2.0), forward=(0, -1, 0), inner_cone_deg=0.0, outer_cone_deg=0.0))
+    d.lights.append(dict(type=LIGHT_DIRECTIONAL, color=(1.0, 0.95, 0.9), intensity=1.5, radius=0.05,
+                         position=(0, 0, 0), forward=tuple(sun_forward(35.0, 60.0)), inner_cone_deg=0.0,
+                         outer_cone_deg=0.0))
+    d.sky = (3.0, 0.1, 1.15, 64)
+    d.camera = look_at_camera((0.0, 3.2, 5.5), (0.0, 0.4, 0.0), fov_y=45.0)
+    d.width, d.height = width, height
+    d.max_ray_bounces = max_ray_bounces
+    d.spp = 16
+    return d
+
+
 def random_rays(n, lo, hi, seed=0, tmin=0.0, tmax=1e4):
     """Rays from random points in the AABB [lo, hi] towards random directions."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -280,6 +280,14 @@ hh_status hh_scene_sky_coefficients(hh_scene scene, float out[30]);
 hh_status hh_sky_fit(const hh_sky_params* params, const float sun_dir[3], float out[30]);
 /* BVH statistics: node and triangle counts over all BLAS + TLAS. */
 hh_status hh_scene_bvh_stats(hh_scene scene, uint64_t out[4]);
+/* Traversal limit counters of every ray traced against `scene` since it was
+ * committed (or last reset): out[0] pushes dropped from a full per-lane stack,
+ * out[1] pushes dropped from a full wave-uniform packet stack, out[2] rays
+ * stopped at the bound on node and leaf visits. Each such ray may have missed
+ * its closest hit or its occluder; all three stay 0 for trees the builders
+ * make. `reset` != 0 clears them after reading. No reference counterpart:
+ * the Vulkan driver's traversal has no such limits to report. */
+hh_status hh_scene_traversal_faults(hh_scene scene, uint64_t out[3], int reset);
 
 /* ---------------------------------------------------------------------------
  * Scene files: glTF 2.0 (.gltf with external or data-URI buffers, or .glb),

@@ -67,6 +67,8 @@ def test_full_scene_schedules_bit_identical(gpu_ctx, full_field, full_scene):
     assert np.array_equal(packets, mega)
     assert s0 == s1 == s2
     assert s0["extension_rays"] > 4 * 1920 * 1080
+    # no ray of the three schedules was cut short by a stack or visit bound
+    assert full_scene.traversal_faults() == dict(lane_stack=0, packet_stack=0, trips=0)
 
 
 def test_full_scene_against_oracle_at_reduced_resolution(gpu_ctx, oracle_mod, full_field, full_scene):
@@ -82,3 +84,4 @@ def test_full_scene_against_oracle_at_reduced_resolution(gpu_ctx, oracle_mod, fu
     img, _ = _render(gpu_ctx, full_scene, small, 8, width=small.width, height=small.height)
     ref = o.render(frames=8)
     assert relmse(img, ref) < 1e-4, relmse(img, ref)
+    assert full_scene.traversal_faults() == dict(lane_stack=0, packet_stack=0, trips=0)
