@@ -529,22 +529,35 @@ def field_for_triangles(seed, target_tris, width, height, max_ray_bounces=8, spp
                            max_ray_bounces=max_ray_bounces, spp=spp)
 
 
-def batched_workload(n=64, seed0=0, min_side=128, max_side=1024, min_tris=10_000, max_tris=2_000_000,
-                     max_ray_bounces=8, spp=16):
-    """The batched workload: n independent scenes and cameras (seeds seed0..),
-    square resolutions log-uniform in [min_side, max_side] and triangle counts
-    log-uniform in [min_tris, max_tris], built with the icosphere-field
-    generator."""
+def batched_workload_params(n=64, seed0=0, min_side=128, max_side=1024, min_tris=10_000, max_tris=2_000_000):
+    """(seed, side, target triangles, camera) of each scene of the batched
+    workload, without building any geometry."""
     rng = np.random.Generator(np.random.PCG64(seed0 + 7919))
     out = []
     for i in range(n):
         side = int(round(math.exp(rng.uniform(math.log(min_side), math.log(max_side)))))
         tris = int(round(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))))
-        d = field_for_triangles(seed0 + i, tris, side, side, max_ray_bounces, spp)
         ang = rng.uniform(0, 2 * np.pi)
         dist = rng.uniform(60.0, 150.0)
-        d.camera = look_at_camera((dist * math.cos(ang), rng.uniform(10.0, 50.0), dist * math.sin(ang)),
-                                  (0.0, 0.0, 0.0), fov_y=60.0, near=1.0, far=1000.0, aperture=0.0)
+        cam = look_at_camera((dist * math.cos(ang), rng.uniform(10.0, 50.0), dist * math.sin(ang)),
+                             (0.0, 0.0, 0.0), fov_y=60.0, near=1.0, far=1000.0, aperture=0.0)
+        out.append((seed0 + i, side, tris, cam))
+    return out
+
+
+def batched_workload(n=64, seed0=0, min_side=128, max_side=1024, min_tris=10_000, max_tris=2_000_000,
+                     max_ray_bounces=8, spp=16, select=None):
+    """The batched workload: n independent scenes and cameras (seeds seed0..),
+    square resolutions log-uniform in [min_side, max_side] and triangle counts
+    log-uniform in [min_tris, max_tris], built with the icosphere-field
+    generator. `select` (indices) builds only those scenes of the n."""
+    params = batched_workload_params(n, seed0, min_side, max_side, min_tris, max_tris)
+    out = []
+    for i, (seed, side, tris, cam) in enumerate(params):
+        if select is not None and i not in select:
+            continue
+        d = field_for_triangles(seed, tris, side, side, max_ray_bounces, spp)
+        d.camera = cam
         out.append(d)
     return out
 
