@@ -79,7 +79,15 @@ struct Json {
     }
     long integer(const char* k, long def) const {
         const Json* v = get(k);
-        return v && v->type == Number ? long(v->num) : def;
+        return v && v->type == Number ? v->as_long() : def;
+    }
+    // The number as a long, or -1 when it is not an integer in
+    // [-2^53, 2^53] (every valid index, count and offset is; converting
+    // a double outside long's range would be undefined).
+    long as_long() const {
+        if (type != Number || !(num >= -9007199254740992.0 && num <= 9007199254740992.0) || num != std::floor(num))
+            return -1;
+        return long(num);
     }
     std::string string(const char* k, const std::string& def = "") const {
         const Json* v = get(k);
@@ -515,10 +523,14 @@ struct Loader {
         const long b = v->integer("buffer", -1);
         if (b < 0 || size_t(b) >= buffers.size()) return fail("bad buffer index");
         const long off = v->integer("byteOffset", 0), n = v->integer("byteLength", -1);
-        if (off < 0 || n < 0 || size_t(off) + size_t(n) > buffers[size_t(b)].size()) return fail("bufferView out of range");
+        const size_t have = buffers[size_t(b)].size();
+        if (off < 0 || n < 0 || size_t(off) > have || size_t(n) > have - size_t(off)) return fail("bufferView out of range");
+        // glTF 2.0: byteStride, when present, is in [4, 252] and a multiple of 4
+        const long st = v->integer("byteStride", 0);
+        if (v->get("byteStride") && (st < 4 || st > 252 || st % 4 != 0)) return fail("bad bufferView byteStride");
         ptr = buffers[size_t(b)].data() + off;
         len = size_t(n);
-        stride = size_t(v->integer("byteStride", 0));
+        stride = size_t(st);
         return true;
     }
 
@@ -549,8 +561,13 @@ struct Loader {
         if (!view(a->integer("bufferView", -1), p, len, stride)) return false;
         const size_t elem = size_t(csize) * comps;
         if (stride == 0) stride = elem;
-        const size_t off = size_t(a->integer("byteOffset", 0));
-        if (count > 0 && off + stride * size_t(count - 1) + elem > len) return fail("accessor out of range");
+        if (stride < elem) return fail("bufferView byteStride below the element size");
+        const long aoff = a->integer("byteOffset", 0);
+        if (aoff < 0 || aoff % csize != 0) return fail("bad accessor byteOffset");
+        const size_t off = size_t(aoff);
+        // off + stride * (count - 1) + elem <= len, without overflow
+        if (count > 0 && (off > len || elem > len - off || size_t(count - 1) > (len - off - elem) / stride))
+            return fail("accessor out of range");
         for (long i = 0; i < count; i++) {
             const uint8_t* e = p + off + stride * size_t(i);
             for (int c = 0; c < comps; c++) {
@@ -958,7 +975,7 @@ struct Loader {
         }
         if (const Json* ch = n->get("children"))
             for (const Json& c : ch->arr)
-                if (!node(long(c.num), g, depth + 1)) return false;
+                if (!node(c.as_long(), g, depth + 1)) return false;
         return true;
     }
 
@@ -970,13 +987,13 @@ struct Loader {
             const Json* sc = element("scenes", si);
             if (!sc) return fail("bad scene index");
             if (const Json* ns = sc->get("nodes"))
-                for (const Json& x : ns->arr) roots.push_back(long(x.num));
+                for (const Json& x : ns->arr) roots.push_back(x.as_long());
         } else if (const Json* nodes = array("nodes")) {
             std::vector<char> child(nodes->arr.size(), 0);
             for (const Json& n : nodes->arr)
                 if (const Json* ch = n.get("children"))
                     for (const Json& c : ch->arr)
-                        if (c.num >= 0 && size_t(c.num) < child.size()) child[size_t(c.num)] = 1;
+                        if (c.as_long() >= 0 && size_t(c.as_long()) < child.size()) child[size_t(c.as_long())] = 1;
             for (size_t i = 0; i < child.size(); i++)
                 if (!child[i]) roots.push_back(long(i));
         }

@@ -309,3 +309,58 @@ def test_errors_are_reported(tmp_path):
         runtime.gltf_load(req, Capture())
     with pytest.raises(runtime.HeliosError, match="JPEG"):
         _png_file(tmp_path, "photo.jpg", b"\xff\xd8\xff\xe0" + b"\x00" * 16)
+
+
+def _one_triangle_doc(acc_extra=None, view_extra=None, blob_len=None):
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32).tobytes()
+    blob = P + b"\x00" * 16
+    view = {"buffer": 0, "byteOffset": 0, "byteLength": len(P)}
+    view.update(view_extra or {})
+    acc = {"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}
+    acc.update(acc_extra or {})
+    return {"asset": {"version": "2.0"},
+            "buffers": [{"byteLength": blob_len if blob_len is not None else len(blob), "uri": data_uri(blob)}],
+            "bufferViews": [view], "accessors": [acc],
+            "meshes": [{"primitives": [{"attributes": {"POSITION": 0}}]}],
+            "nodes": [{"mesh": 0}], "scenes": [{"nodes": [0]}]}
+
+
+@pytest.mark.parametrize("acc_extra,view_extra,match", [
+    ({"byteOffset": -8}, None, "byteOffset"),                          # negative accessor offset
+    ({"byteOffset": 2}, None, "byteOffset"),                           # not a multiple of the component size
+    ({"byteOffset": 1 << 62}, None, "byteOffset"),                     # beyond 2^53: not an exact integer
+    ({"byteOffset": (1 << 52) + 4}, None, "out of range"),             # would wrap off + stride * (count - 1)
+    (None, {"byteStride": 1 << 40}, "byteStride"),                     # huge stride
+    (None, {"byteStride": -12}, "byteStride"),                         # negative stride
+    (None, {"byteStride": 254}, "byteStride"),                         # above 252
+    (None, {"byteStride": 6}, "byteStride"),                           # not a multiple of 4
+    (None, {"byteStride": 8}, "element size"),                         # below the 12-B VEC3 element
+    ({"count": 1e300}, None, "count"),                                 # beyond long's range
+    ({"count": 2.5}, None, "count"),                                   # not an integer
+    (None, {"byteOffset": -4}, "bufferView out of range"),             # negative view offset
+    (None, {"byteLength": 1 << 62}, "bufferView out of range"),        # offset + length wraps
+    ({"count": 4}, None, "accessor out of range"),                     # one element past the view
+])
+def test_malformed_accessors_are_refused(tmp_path, acc_extra, view_extra, match):
+    """Offsets, strides and counts outside glTF 2.0's rules (or that would
+    overflow the range check) are refused before any byte is read."""
+    path = tmp_path / "bad_accessor.gltf"
+    path.write_text(json.dumps(_one_triangle_doc(acc_extra, view_extra)))
+    with pytest.raises(runtime.HeliosError, match=match):
+        runtime.gltf_load(path, Capture())
+
+
+def test_valid_stride_and_offset_are_read(tmp_path):
+    """A strided view (stride 16 > element 12) with an accessor offset reads
+    the elements it names."""
+    P = np.zeros((4, 4), np.float32)
+    P[1:, :3] = [[0, 0, 0], [2, 0, 0], [0, 3, 0]]
+    blob = P.tobytes()
+    doc = _one_triangle_doc({"byteOffset": 16}, {"byteStride": 16, "byteLength": len(blob)})
+    doc["buffers"] = [{"byteLength": len(blob), "uri": data_uri(blob)}]
+    path = tmp_path / "strided.gltf"
+    path.write_text(json.dumps(doc))
+    cap = Capture()
+    runtime.gltf_load(path, cap)
+    v = cap.meshes[0][0]
+    assert np.allclose(v["position"][:, :3], [[0, 0, 0], [2, 0, 0], [0, 3, 0]])
