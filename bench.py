@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--refill-lanes", type=int, default=24)
     p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
     p.add_argument("--bvh-builder", type=int, default=1, help="0 LBVH on the GPU, 1 binned SAH on the host")
+    p.add_argument("--continuation-depth", type=int, default=0,
+                   help="bounces traced as wavefront launches before one continuation launch finishes every path "
+                        "(0 = every bounce as wavefront launches)")
     p.add_argument("--megakernel", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -408,7 +411,8 @@ def main():
     ctx = runtime.Context(0)
     ctx.set_tuning(bvh_width=a.bvh_width, extend_block=a.block, persistent_blocks_per_cu=a.blocks_per_cu,
                    frames_in_flight=a.frames_in_flight, bvh_compressed=a.bvh_compressed,
-                   refill_lanes=a.refill_lanes, traversal=a.traversal, bvh_builder=a.bvh_builder)
+                   refill_lanes=a.refill_lanes, traversal=a.traversal, bvh_builder=a.bvh_builder,
+                   continuation_depth=a.continuation_depth)
     node_bytes = 64 if (a.bvh_compressed and a.bvh_width == 4) else NODE_BYTES[a.bvh_width]
     t0 = time.perf_counter()
     scene = runtime.Scene(ctx, desc)
@@ -489,6 +493,7 @@ def main():
             "refill_lanes": a.refill_lanes, "traversal": "if-if" if a.traversal else "while-while",
             "bvh_builder": "binned SAH (host)" if a.bvh_builder else "LBVH (GPU)",
             "integrator": "megakernel" if a.megakernel else "wavefront+hipGraph",
+            "continuation_depth": a.continuation_depth,
         },
         "gsample_bounces_per_s": round(gsb, 4),
         "shadow_rays_per_frame": stats["shadow_rays"] / max(stats["frames"], 1),

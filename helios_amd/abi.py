@@ -82,7 +82,7 @@ class Tuning(C.Structure):
     _fields_ = [("bvh_width", C.c_uint32), ("extend_block", C.c_uint32), ("shade_block", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("persistent_blocks_per_cu", C.c_uint32),
                 ("frames_in_flight", C.c_uint32), ("bvh_compressed", C.c_uint32), ("refill_lanes", C.c_uint32),
-                ("traversal", C.c_uint32), ("bvh_builder", C.c_uint32)]
+                ("traversal", C.c_uint32), ("bvh_builder", C.c_uint32), ("continuation_depth", C.c_uint32)]
 
 
 assert C.sizeof(Vertex) == 80

@@ -390,7 +390,9 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->refill_lanes >= 1 && t->refill_lanes <= 64, "refill_lanes must be 1..64");
     HH_ARG(t->traversal <= 1, "traversal must be 0 (while-while) or 1 (if-if)");
     HH_ARG(t->bvh_builder <= 1, "bvh_builder must be 0 (LBVH) or 1 (SAH)");
+    HH_ARG(t->continuation_depth <= 64, "continuation_depth must be 0..64");
     ctx->tune.bvh_builder = t->bvh_builder;
+    ctx->tune.continuation_depth = t->continuation_depth;
     ctx->tune.refill_lanes = t->refill_lanes;
     ctx->tune.traversal = (int)t->traversal;
     ctx->tune.frames_in_flight = t->frames_in_flight;
@@ -496,6 +498,7 @@ hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t) {
     t->refill_lanes = ctx->tune.refill_lanes;
     t->traversal = (uint32_t)ctx->tune.traversal;
     t->bvh_builder = ctx->tune.bvh_builder;
+    t->continuation_depth = ctx->tune.continuation_depth;
     return HH_OK;
 }
 

@@ -911,7 +911,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
                                                    uint32_t* __restrict__ head, uint32_t seg, uint32_t refill,
                                                    const float4* __restrict__ Tin, const uint2* __restrict__ rngs,
                                                    float4* __restrict__ Lout, unsigned long long* __restrict__ stats,
-                                                   uint32_t* __restrict__ spill) {
+                                                   uint32_t* __restrict__ spill, uint32_t depth0) {
     __shared__ uint32_t stack[kLdsStack * BS];
     const LaneStack stk{to_lds(&stack[threadIdx.x]), BS, to_global(spill + blockIdx.x * BS + threadIdx.x), gridDim.x * BS};
     QueueCursor qc(count, head, seg, refill);
@@ -935,7 +935,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
             const uint2 rs = rngs[slot];
             rng = RNG{rs.x, rs.y};
             rflags = HH_TRACE_OPAQUE_BIT;
-            depth = 1;
+            depth = depth0;
             active = true;
         }
         if (__ballot(active) == 0ull) break;
@@ -1387,17 +1387,20 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
         k_generate<BS><<<grid, BS, 0, st>>>(d_views, nv, total, wf.q_o[0], wf.q_d[0], wf.q_slot[0], wf.L, wf.rng, ctr, seg,
                                             max_bounces == 0 ? 1u : 0u);
     }
-    if (mega) {
-        // Camera rays as wavefront launches (one large coherent queue), then
-        // the remaining bounces of every path in one launch, so a frame is
-        // not held by the tails of seven more rounds of launches.
-        if (max_bounces > 0) bounce(0);
-        if (max_bounces > 1) {
-            ProfScope ps(pf, PK_MEGAKERNEL, st);
-            k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[1], wf.q_d[1], wf.q_slot[1],
-                                                     ctr + ctr_index(kCtrExt1, 0), ctr + ctr_index(kCtrExtHead, 0), seg,
-                                                     refill_paths, wf.T, wf.rng, wf.L, wf.stats, wf.spill);
-        }
+    // The first `cont` bounces as wavefront launches (large coherent queues),
+    // then the remaining bounces of every path in one continuation launch, so
+    // a pass is not held by the latency tails of the small deep-bounce
+    // launches. cont = 0: every bounce as wavefront launches.
+    // (the counting variant traces every bounce as wavefront launches: it
+    // counts the fetches of the per-lane traversal kernels)
+    const uint32_t cont = COUNT ? 0u : (cfg.tune.continuation_depth ? cfg.tune.continuation_depth : (mega ? 1u : 0u));
+    if (cont > 0 && cont < max_bounces) {
+        for (uint32_t d = 0; d < cont; d++) bounce(d);
+        const int q = (int)(cont & 1u);  // bounce(cont - 1) queued its extension rays there
+        ProfScope ps(pf, PK_MEGAKERNEL, st);
+        k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[q], wf.q_d[q], wf.q_slot[q],
+                                                 ctr + ctr_index(kCtrExt0 + q, 0), ctr + ctr_index(kCtrExtHead, 0), seg,
+                                                 refill_paths, wf.T, wf.rng, wf.L, wf.stats, wf.spill, cont);
     } else {
         for (uint32_t d = 0; d < max_bounces; d++) bounce(d);
     }

@@ -119,6 +119,7 @@ struct Tuning {
     uint32_t bvh_builder = 1;   // 0: LBVH on the GPU; 1: binned SAH on the host
     uint32_t refill_lanes = 24;  // idle lanes that make a traversal wave take new rays
     int traversal = 0;           // 0 while-while, 1 if-if step shape
+    uint32_t continuation_depth = 0;  // bounces before the continuation launch (0 = none)
     int layout() const { return bvh_compressed && bvh_width == 4 ? kLayoutQ4 : bvh_width; }
 };
 

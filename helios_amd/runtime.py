@@ -155,9 +155,11 @@ class Context:
         check(lib().hh_sync(self.h))
 
     def set_tuning(self, bvh_width=4, extend_block=256, queue_capacity=0, persistent_blocks_per_cu=8,
-                   frames_in_flight=4, bvh_compressed=False, refill_lanes=24, traversal=0, bvh_builder=1):
+                   frames_in_flight=4, bvh_compressed=False, refill_lanes=24, traversal=0, bvh_builder=1,
+                   continuation_depth=0):
         t = abi.Tuning(bvh_width, extend_block, extend_block, queue_capacity, persistent_blocks_per_cu,
-                       frames_in_flight, int(bool(bvh_compressed)), refill_lanes, traversal, bvh_builder)
+                       frames_in_flight, int(bool(bvh_compressed)), refill_lanes, traversal, bvh_builder,
+                       continuation_depth)
         check(lib().hh_context_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
@@ -166,7 +168,7 @@ class Context:
         return dict(bvh_width=t.bvh_width, extend_block=t.extend_block, queue_capacity=t.queue_capacity,
                     persistent_blocks_per_cu=t.persistent_blocks_per_cu, frames_in_flight=t.frames_in_flight,
                     bvh_compressed=bool(t.bvh_compressed), refill_lanes=t.refill_lanes, traversal=t.traversal,
-                    bvh_builder=t.bvh_builder)
+                    bvh_builder=t.bvh_builder, continuation_depth=t.continuation_depth)
 
     def stream(self):
         s = C.c_void_p()

@@ -518,6 +518,13 @@ typedef struct hh_tuning {
      * the role of PREFER_FAST_TRACE in mesh.cpp:109; the default). Hits are
      * identical either way (ties resolve by instance, geometry, primitive). */
     uint32_t bvh_builder;
+    /* Paths still alive after this many bounces traced as wavefront launches
+     * finish in one continuation launch, in which a lane whose path ends takes
+     * the next queued path (deep bounces hold few rays, and a launch per bounce
+     * pays its latency tail each time). 0 = every bounce as wavefront launches
+     * (HH_FLAG_MEGAKERNEL then continues after the camera rays' bounce, i.e.
+     * 1); 1..64 otherwise. Results are identical. */
+    uint32_t continuation_depth;
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
 hh_status hh_context_get_tuning(hh_context ctx, hh_tuning* t);

@@ -176,6 +176,23 @@ def test_megakernel_equals_wavefront(gpu_ctx, small_field, field_pair, bounces):
     assert sa["extension_rays"] == sb["extension_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
 
 
+@pytest.mark.parametrize("cont", [2, 3, 7])
+def test_continuation_depth_equals_wavefront(gpu_ctx, small_field, field_pair, cont):
+    """Wavefront launches for the first `cont` bounces, then one continuation
+    launch for the rest of every path: the same image, bit for bit, and the
+    same ray counts, with several frames per pass."""
+    g, _ = field_pair
+    gpu_ctx.set_tuning(frames_in_flight=3)
+    try:
+        a, sa = _render_gpu(gpu_ctx, g, small_field, frames=5)
+        gpu_ctx.set_tuning(frames_in_flight=3, continuation_depth=cont)
+        b, sb = _render_gpu(gpu_ctx, g, small_field, frames=5)
+    finally:
+        gpu_ctx.set_tuning()
+    assert np.array_equal(a, b)
+    assert sa == sb
+
+
 def test_tonemap_matches_oracle(gpu_ctx, oracle_mod):
     from helios_amd import runtime
     rng = np.random.default_rng(0)
