@@ -389,7 +389,7 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 256, "frames_in_flight must be 1..256");
     HH_ARG(t->refill_lanes >= 1 && t->refill_lanes <= 64, "refill_lanes must be 1..64");
     HH_ARG(t->traversal <= 1, "traversal must be 0 (while-while) or 1 (if-if)");
-    HH_ARG(t->bvh_builder <= 1, "bvh_builder must be 0 (LBVH) or 1 (SAH)");
+    HH_ARG(t->bvh_builder <= 2, "bvh_builder must be 0 (LBVH), 1 (binned SAH on the host) or 2 (PLOC)");
     HH_ARG(t->continuation_depth <= 64, "continuation_depth must be 0..64");
     ctx->tune.bvh_builder = t->bvh_builder;
     ctx->tune.continuation_depth = t->continuation_depth;
@@ -881,7 +881,7 @@ hh_status hh_scene_commit(hh_scene s) {
             e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup, d_tris, d_shade_soup, d_shade,
                            ctx->tune.bvh_builder == 1 ? soup.data() : nullptr, tri_base, d_nodes,
                            node_base, ctx->bs, &nn,
-                           &blas_box[g * 6]);
+                           &blas_box[g * 6], ctx->tune.bvh_builder == 2);
         if (e != hipSuccess) {
             st = hip_fail(e, "BLAS build");
             break;
@@ -951,7 +951,7 @@ hh_status hh_scene_commit(hh_scene s) {
         tlas_nodes = 1;
     } else {
         hipError_t e = build_tlas(ctx->stream, W, (uint32_t)s->instances.size(), ilo.data(), ihi.data(), d_nodes, node_base, ctx->bs,
-                                  &tlas_nodes, ctx->tune.bvh_builder == 1);
+                                  &tlas_nodes, ctx->tune.bvh_builder == 1, ctx->tune.bvh_builder == 2);
         if (e != hipSuccess) return hip_fail(e, "TLAS build");
     }
     s->bvh_stats[2] = tlas_nodes;

@@ -365,6 +365,165 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// PLOC: parallel locally-ordered clustering (Meister and Bittner, "Parallel
+// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 24(3), 2018). The clusters start as the primitives in Morton
+// order; every round each cluster finds its nearest neighbour among the
+// kPlocRadius clusters on either side (distance = surface area of the union
+// box), mutual nearest neighbours merge into a new node that takes the lower
+// position, and the cluster array is compacted, until one cluster is left.
+// Ties are broken by the neighbour's position, which orders the candidate
+// pairs of every cluster by one total order on pairs (area, lower position,
+// higher position); the smallest pair of a round is then mutual, so every
+// round merges at least one pair.
+//
+// Node ids: leaf k = k-th primitive in Morton order (0..n-1); internal nodes
+// n..2n-2 in creation order, so the root is 2n-2. A merged node's children
+// are (cluster at the lower position, cluster at the higher position).
+// ---------------------------------------------------------------------------
+constexpr int kPlocRadius = 16;
+constexpr int kPlocBlock = 256;
+
+__global__ void k_ploc_init(const float4* __restrict__ lo, const float4* __restrict__ hi, const uint32_t* __restrict__ sorted,
+                            uint32_t n, float4* __restrict__ blo, float4* __restrict__ bhi, uint32_t* __restrict__ clusters,
+                            uint32_t* __restrict__ count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = sorted[i];
+    float4 l = lo[p], h = hi[p];
+    blo[i] = make_float4(l.x, l.y, l.z, 0.f);
+    bhi[i] = make_float4(h.x, h.y, h.z, 0.f);
+    clusters[i] = i;
+    count[i] = 1u;
+}
+
+__device__ inline float union_half_area(float ax, float ay, float az, float bx, float by, float bz, float cx, float cy, float cz,
+                                        float dx, float dy, float dz) {
+    // boxes (a lo, b hi) and (c lo, d hi)
+    float ex = fmaxf(bx, dx) - fminf(ax, cx), ey = fmaxf(by, dy) - fminf(ay, cy), ez = fmaxf(bz, dz) - fminf(az, cz);
+    float s = ex * ey + ey * ez + ez * ex;
+    return s == s ? s : INFINITY;   // a NaN box never wins a tie-free comparison otherwise
+}
+
+// nn[i] = the position j (|i - j| <= kPlocRadius, j != i) minimizing
+// (union area, j). The block stages the boxes of its positions and of the
+// kPlocRadius positions on either side in LDS.
+__global__ void __launch_bounds__(kPlocBlock) k_ploc_nn(const uint32_t* __restrict__ clusters, uint32_t m,
+                                                        const float4* __restrict__ blo, const float4* __restrict__ bhi,
+                                                        uint32_t* __restrict__ nn) {
+    constexpr int S = kPlocBlock + 2 * kPlocRadius;
+    __shared__ float box[6][S];
+    const int base = (int)(blockIdx.x * kPlocBlock);
+    for (int k = threadIdx.x; k < S; k += kPlocBlock) {
+        int idx = base - kPlocRadius + k;
+        if (idx >= 0 && idx < (int)m) {
+            uint32_t c = clusters[idx];
+            float4 l = blo[c], h = bhi[c];
+            box[0][k] = l.x; box[1][k] = l.y; box[2][k] = l.z;
+            box[3][k] = h.x; box[4][k] = h.y; box[5][k] = h.z;
+        }
+    }
+    __syncthreads();
+    const int i = base + (int)threadIdx.x;
+    if (i >= (int)m) return;
+    const int me = (int)threadIdx.x + kPlocRadius;
+    const float ax = box[0][me], ay = box[1][me], az = box[2][me], bx = box[3][me], by = box[4][me], bz = box[5][me];
+    float best = INFINITY;
+    int bj = -1;
+    const int j0 = max(0, i - kPlocRadius), j1 = min((int)m - 1, i + kPlocRadius);
+    for (int j = j0; j <= j1; j++) {
+        if (j == i) continue;
+        const int s = j - base + kPlocRadius;
+        float d = union_half_area(ax, ay, az, bx, by, bz, box[0][s], box[1][s], box[2][s], box[3][s], box[4][s], box[5][s]);
+        if (bj < 0 || d < best) {   // j ascends: an equal area keeps the lower position
+            best = d;
+            bj = j;
+        }
+    }
+    nn[i] = (uint32_t)bj;
+}
+
+// Per position: bit 0 = the cluster stays (it is not the higher partner of a
+// mutual pair), bit 32 = it is the lower partner of a mutual pair (a merge).
+__global__ void k_ploc_flags(const uint32_t* __restrict__ nn, uint32_t m, unsigned long long* __restrict__ flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t j = nn[i];
+    bool mutual = nn[j] == i;
+    unsigned long long stay = (mutual && j < i) ? 0ull : 1ull;
+    unsigned long long merge = (mutual && i < j) ? 1ull : 0ull;
+    flags[i] = stay | (merge << 32);
+}
+
+// Writes the next cluster array; a merging cluster becomes the new node
+// next_id + (merges before it), with its union box and leaf count.
+__global__ void k_ploc_compact(const uint32_t* __restrict__ clusters, const uint32_t* __restrict__ nn, uint32_t m,
+                               const unsigned long long* __restrict__ flags, const unsigned long long* __restrict__ scan,
+                               uint32_t n, uint32_t next_id, float4* __restrict__ blo, float4* __restrict__ bhi,
+                               int2* __restrict__ child, uint32_t* __restrict__ parent, uint32_t* __restrict__ count,
+                               uint32_t* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    unsigned long long f = flags[i];
+    if (!(f & 1ull)) return;
+    unsigned long long sc = scan[i];
+    uint32_t pos = (uint32_t)sc;
+    uint32_t c = clusters[i];
+    if (f >> 32) {
+        uint32_t cj = clusters[nn[i]];
+        uint32_t id = next_id + (uint32_t)(sc >> 32);
+        float4 al = blo[c], ah = bhi[c], bl = blo[cj], bh = bhi[cj];
+        blo[id] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
+        bhi[id] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+        child[id - n] = make_int2((int)c, (int)cj);
+        parent[c] = id;
+        parent[cj] = id;
+        count[id] = count[c] + count[cj];
+        out[pos] = id;
+    } else {
+        out[pos] = c;
+    }
+}
+
+// Position of every node's first leaf in the depth-first leaf order (lower
+// child first): the leaf counts of the lower siblings of the node's
+// ancestors (and of the node itself) that it lies to the right of.
+__global__ void k_ploc_offsets(uint32_t n, const int2* __restrict__ child, const uint32_t* __restrict__ parent,
+                               const uint32_t* __restrict__ count, uint32_t* __restrict__ off) {
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t root = 2 * n - 2;
+    if (x > root) return;
+    uint32_t o = 0, y = x;
+    while (y != root) {
+        uint32_t p = parent[y];
+        int2 c = child[p - n];
+        if ((uint32_t)c.y == y) o += count[(uint32_t)c.x];
+        y = p;
+    }
+    off[x] = o;
+}
+
+// The clustered tree in the radix tree's form: internal node x -> 2n-2-x
+// (root 0), leaf x -> binary ref n-1+off[x], and the leaf order.
+__global__ void k_ploc_to_radix(uint32_t n, const int2* __restrict__ pchild, const uint32_t* __restrict__ pparent,
+                                const uint32_t* __restrict__ pcount, const uint32_t* __restrict__ off,
+                                const uint32_t* __restrict__ sorted, uint32_t* __restrict__ order, int2* __restrict__ childs,
+                                int2* __restrict__ range, uint32_t* __restrict__ parent) {
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t root = 2 * n - 2;
+    if (x > root) return;
+    auto ref = [&](uint32_t y) { return y < n ? (n - 1 + off[y]) : (root - y); };
+    if (x < n) {
+        order[off[x]] = sorted[x];
+    } else {
+        int2 c = pchild[x - n];
+        childs[root - x] = make_int2((int)ref((uint32_t)c.x), (int)ref((uint32_t)c.y));
+        range[root - x] = make_int2((int)off[x], (int)(off[x] + pcount[x] - 1));
+    }
+    if (x != root) parent[ref(x)] = root - pparent[x];
+}
+
 inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 }  // namespace
@@ -520,10 +679,45 @@ template <int W>
 static hipError_t build_wide(hipStream_t st, uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_cbounds,
                              WideNode<W>* d_nodes, uint32_t node_base, uint32_t leaf_base, int leaf_size, int leaf_is_prim,
                              uint32_t* d_order, BuildScratch& sc, uint32_t* num_nodes, float root_box[6],
-                             const float* h_lo = nullptr, const float* h_hi = nullptr) {
+                             const float* h_lo = nullptr, const float* h_hi = nullptr, bool ploc = false) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
-    if (h_lo) {
+    if (ploc && n > 1) {
+        HH_TRY(sc.ensure_ploc(n));
+        k_morton<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, n, d_cbounds, sc.keys_in, sc.vals_in);
+        size_t tb = sc.sort_bytes;
+        HH_TRY(hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.keys_in, sc.keys_out, sc.vals_in, sc.p_sorted, (int)n, 0,
+                                                  30, st));
+        // PLOC boxes live in blo/bhi (ids 0..2n-2); the refit below rewrites
+        // them in the radix tree's numbering.
+        k_ploc_init<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, sc.p_sorted, n, sc.blo, sc.bhi, sc.p_cl[0], sc.p_count);
+        uint32_t m = n, next_id = n, cur = 0;
+        unsigned long long* tail = sc.p_tail;
+        while (m > 1) {
+            k_ploc_nn<<<blocks(m, kPlocBlock), kPlocBlock, 0, st>>>(sc.p_cl[cur], m, sc.blo, sc.bhi, sc.p_nn);
+            k_ploc_flags<<<blocks(m, B), B, 0, st>>>(sc.p_nn, m, sc.p_flags);
+            size_t sb = sc.scan_bytes;
+            HH_TRY(hipcub::DeviceScan::ExclusiveSum(sc.scan_tmp, sb, sc.p_flags, sc.p_scan, (int)m, st));
+            k_ploc_compact<<<blocks(m, B), B, 0, st>>>(sc.p_cl[cur], sc.p_nn, m, sc.p_flags, sc.p_scan, n, next_id, sc.blo,
+                                                       sc.bhi, sc.p_child, sc.p_parent, sc.p_count, sc.p_cl[cur ^ 1]);
+            HH_TRY(hipMemcpyAsync(&tail[0], sc.p_flags + (m - 1), 8, hipMemcpyDeviceToHost, st));
+            HH_TRY(hipMemcpyAsync(&tail[1], sc.p_scan + (m - 1), 8, hipMemcpyDeviceToHost, st));
+            HH_TRY(hipStreamSynchronize(st));
+            const unsigned long long tot = tail[0] + tail[1];
+            const uint32_t stay = (uint32_t)tot, merges = (uint32_t)(tot >> 32);
+            if (merges == 0 || stay + merges != m) return hipErrorUnknown;   // every round merges a pair
+            m = stay;
+            next_id += merges;
+            cur ^= 1;
+        }
+        if (next_id != 2 * n - 1) return hipErrorUnknown;
+        k_ploc_offsets<<<blocks(2 * n - 1, B), B, 0, st>>>(n, sc.p_child, sc.p_parent, sc.p_count, sc.p_off);
+        k_ploc_to_radix<<<blocks(2 * n - 1, B), B, 0, st>>>(n, sc.p_child, sc.p_parent, sc.p_count, sc.p_off, sc.p_sorted, d_order,
+                                                           sc.childs, sc.range, sc.parent);
+        k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
+    } else if (ploc) {
+        k_fill_u32<<<1, 1, 0, st>>>(d_order, 0u, 1);
+    } else if (h_lo) {
         // binned SAH on the host (the boxes are the host copies of d_lo / d_hi)
         SahTree t;
         std::vector<SahPrim> prims;
@@ -576,6 +770,41 @@ static hipError_t build_wide(hipStream_t st, uint32_t n, const float4* d_lo, con
     return hipSuccess;
 }
 
+hipError_t BuildScratch::ensure_ploc(uint32_t n) {
+    if (n <= p_cap) return hipSuccess;
+    release_ploc();
+    p_cap = std::max<uint32_t>(n, 1024);
+    size_t N = p_cap;
+    HH_TRY(hipMalloc(&p_sorted, N * 4));
+    HH_TRY(hipMalloc(&p_cl[0], N * 4));
+    HH_TRY(hipMalloc(&p_cl[1], N * 4));
+    HH_TRY(hipMalloc(&p_nn, N * 4));
+    HH_TRY(hipMalloc(&p_flags, N * 8));
+    HH_TRY(hipMalloc(&p_scan, N * 8));
+    HH_TRY(hipMalloc(&p_child, N * sizeof(int2)));
+    HH_TRY(hipMalloc(&p_parent, 2 * N * 4));
+    HH_TRY(hipMalloc(&p_count, 2 * N * 4));
+    HH_TRY(hipMalloc(&p_off, 2 * N * 4));
+    HH_TRY(hipHostMalloc(&p_tail, 16));
+    scan_bytes = 0;
+    HH_TRY(hipcub::DeviceScan::ExclusiveSum((void*)nullptr, scan_bytes, p_flags, p_scan, (int)N));
+    HH_TRY(hipMalloc(&scan_tmp, scan_bytes + 256));
+    return hipSuccess;
+}
+
+void BuildScratch::release_ploc() {
+    void* ps[] = {p_sorted, p_cl[0], p_cl[1], p_nn, p_flags, p_scan, p_child, p_parent, p_count, p_off, scan_tmp};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (p_tail) (void)hipHostFree(p_tail);
+    p_sorted = p_cl[0] = p_cl[1] = p_nn = p_parent = p_count = p_off = nullptr;
+    p_flags = p_scan = p_tail = nullptr;
+    p_child = nullptr;
+    scan_tmp = nullptr;
+    scan_bytes = 0;
+    p_cap = 0;
+}
+
 hipError_t BuildScratch::ensure(uint32_t n) {
     if (n <= cap) return hipSuccess;
     release();
@@ -620,7 +849,7 @@ void BuildScratch::release() {
 // d_node_pool + node_base (global indices in child references).
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
                       const float4* d_shade_src, float4* d_shade_pool, const float4* h_tri_src, uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
-                      float root_box[6]) {
+                      float root_box[6], bool ploc) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
     k_init_bounds<<<1, 64, 0, st>>>(sc.cbounds);
@@ -644,10 +873,10 @@ hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, cons
     hipError_t e;
     if (width == 8)
         e = build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base,
-                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph);
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph, ploc);
     else
         e = build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base,
-                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph);
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph, ploc);
     HH_TRY(e);
     k_gather_tris<<<blocks(n, B), B, 0, st>>>(d_tri_src, order, n, d_tri_pool + 3 * (size_t)tri_base);
     k_gather_shade<<<blocks(n, B), B, 0, st>>>(d_shade_src, order, n, d_shade_pool + 4 * (size_t)tri_base);
@@ -662,7 +891,7 @@ hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_
 
 // TLAS over instance AABBs (host arrays, n >= 1); leaves hold one instance.
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
-                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah) {
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah, bool ploc) {
     HH_TRY(sc.ensure(n));
     std::vector<float4> lo(n), hi(n);
     for (uint32_t i = 0; i < n; i++) {
@@ -680,9 +909,9 @@ hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, 
     const float* ph = sah ? h_hi : nullptr;
     if (width == 8)
         return build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base, 0, 1,
-                             1, order, sc, num_nodes, rb, pl, ph);
+                             1, order, sc, num_nodes, rb, pl, ph, ploc);
     return build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base, 0, 1, 1,
-                         order, sc, num_nodes, rb, pl, ph);
+                         order, sc, num_nodes, rb, pl, ph, ploc);
 }
 
 }  // namespace hh

@@ -24,18 +24,28 @@ struct BuildScratch {
     float4 *blo = nullptr, *bhi = nullptr, *aabb_lo = nullptr, *aabb_hi = nullptr;
     void* sort_tmp = nullptr;
     size_t sort_bytes = 0;
+    // clustering (PLOC) state, allocated on first use
+    uint32_t p_cap = 0;
+    uint32_t *p_sorted = nullptr, *p_cl[2] = {nullptr, nullptr}, *p_nn = nullptr;
+    uint32_t *p_parent = nullptr, *p_count = nullptr, *p_off = nullptr;
+    unsigned long long *p_flags = nullptr, *p_scan = nullptr, *p_tail = nullptr;   // p_tail: pinned host
+    int2* p_child = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
     hipError_t ensure(uint32_t n);
+    hipError_t ensure_ploc(uint32_t n);
     void release();
+    void release_ploc();
     uint32_t* vals_out_tmp(uint32_t) { return order; }
-    ~BuildScratch() { release(); }
+    ~BuildScratch() { release(); release_ploc(); }
 };
 
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
                       const float4* d_shade_src, float4* d_shade_pool, const float4* h_tri_src,
                       uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
-                      float root_box[6]);
+                      float root_box[6], bool ploc);
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
-                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah);
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah, bool ploc);
 hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_t n);
 
 // Ray queues are split into kQShards segments, each with its own length and
@@ -116,7 +126,7 @@ struct Tuning {
     int leaf_size = 4;
     uint32_t frames_in_flight = 4;
     bool bvh_compressed = false;
-    uint32_t bvh_builder = 1;   // 0: LBVH on the GPU; 1: binned SAH on the host
+    uint32_t bvh_builder = 1;   // 0: LBVH on the GPU; 1: binned SAH on the host; 2: PLOC on the GPU
     uint32_t refill_lanes = 24;  // idle lanes that make a traversal wave take new rays
     int traversal = 0;           // 0 while-while, 1 if-if step shape
     uint32_t continuation_depth = 0;  // bounces before the continuation launch (0 = none)

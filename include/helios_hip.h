@@ -515,7 +515,8 @@ typedef struct hh_tuning {
     uint32_t traversal;
     /* BLAS / TLAS builder, used at the next scene commit: 0 = LBVH on the GPU
      * (Morton order, fast build), 1 = binned SAH on the host (fast trace,
-     * the role of PREFER_FAST_TRACE in mesh.cpp:109; the default). Hits are
+     * the role of PREFER_FAST_TRACE in mesh.cpp:109; the default), 2 = PLOC
+     * clustering on the GPU (Meister and Bittner 2018; radius 16). Hits are
      * identical either way (ties resolve by instance, geometry, primitive). */
     uint32_t bvh_builder;
     /* Paths still alive after this many bounces traced as wavefront launches

@@ -228,11 +228,14 @@ def test_progressive_protocol(gpu_ctx, cornell, cornell_pair):
 
 
 @pytest.mark.parametrize("layout", [dict(bvh_width=8), dict(bvh_width=4, bvh_compressed=True),
-                                    dict(bvh_builder=0), dict(bvh_builder=0, bvh_width=8)])
+                                    dict(bvh_builder=0), dict(bvh_builder=0, bvh_width=8),
+                                    dict(bvh_builder=2), dict(bvh_builder=2, bvh_width=8),
+                                    dict(bvh_builder=2, bvh_compressed=True)])
 def test_bvh_layouts_bit_identical(gpu_ctx, small_field, field_pair, oracle_mod, layout):
-    """BVH8, the quantized BVH4 and the LBVH trees return the same hits as
-    the oracle and the same images as the binned-SAH float BVH4 (traversal
-    results are independent of the tree's shape and visit order)."""
+    """BVH8, the quantized BVH4, the LBVH and the clustered (PLOC) trees
+    return the same hits as the oracle and the same images as the binned-SAH
+    float BVH4 (traversal results are independent of the tree's shape and
+    visit order)."""
     from helios_amd import runtime
     g4, o = field_pair
     ref_img, _ = _render_gpu(gpu_ctx, g4, small_field, frames=3)
