@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -335,12 +336,52 @@ extern "C" {
 
 const char* hh_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// Scratch of hh_trace_rays_noalloc: one fixed device array per GPU instead of
+// allocations. It holds the traversal-stack spill words of kNoAllocMaxRays
+// lanes (kSpillStackDepth words each, strided by lane) followed by the raw hit
+// records (float4 t,u,v,bits(instance) and two ids per ray). Every entry is
+// written before it is read within one call, so nothing needs clearing; calls
+// on one GPU share the array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoAllocMaxRays = HH_TRACE_NOALLOC_MAX_RAYS;
+constexpr size_t kNoAllocSpillBytes = size_t(kNoAllocMaxRays) * kSpillStackDepth * sizeof(uint32_t);
+constexpr size_t kNoAllocHitBytes = size_t(kNoAllocMaxRays) * sizeof(float4);
+constexpr size_t kNoAllocIdBytes = size_t(kNoAllocMaxRays) * 2 * sizeof(uint32_t);
+constexpr size_t kNoAllocScratchBytes = kNoAllocSpillBytes + kNoAllocHitBytes + kNoAllocIdBytes;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static_assert(kNoAllocMaxRays % 256 == 0, "a k_trace_rays grid of up to kNoAllocMaxRays rays fits the spill area");
+static_assert(kNoAllocSpillBytes % 256 == 0 && kNoAllocHitBytes % 256 == 0, "scratch sections stay aligned");
+static_assert(sizeof(hh_hit) == 24 && sizeof(hh_ray) == 32, "hh_hit / hh_ray layouts");
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current. Host
+// code never passes the array's name itself: on the host it is a stand-in, not
+// a device address.
+static hh_status scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(HH_ERR_LIMIT_EXCEEDED, "device ordinal %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return HH_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = ptr;
+    return HH_OK;
+}
+
 hh_status hh_context_create(int device_ordinal, hh_context* out) {
     HH_ARG(out, "out is null");
     int n = 0;
     HH_HIP(hipGetDeviceCount(&n));
     HH_ARG(device_ordinal >= 0 && device_ordinal < n, "device ordinal out of range");
     HH_HIP(hipSetDevice(device_ordinal));
+    {
+        hh_status st = scratch_init(device_ordinal);
+        if (st != HH_OK) return st;
+    }
     hh_context c = new (std::nothrow) hh_context_t();
     if (!c) return fail(HH_ERR_OUT_OF_MEMORY, "context");
     c->device = device_ordinal;
@@ -1808,6 +1849,34 @@ hh_status hh_trace_rays(hh_scene s, const hh_ray* rays, uint32_t n, uint32_t fla
     HH_HIP(hipMemcpyAsync(ids.data(), d_ids, size_t(n) * 8, hipMemcpyDeviceToHost, ctx->stream));
     HH_HIP(hipStreamSynchronize(ctx->stream));
     to_hits(hit, ids, out, n);
+    return HH_OK;
+}
+
+hh_status hh_trace_rays_noalloc(hh_scene s, const hh_ray* d_rays, uint32_t n, uint32_t flags, hh_hit* d_hits) {
+    HH_ARG(d_rays && d_hits, "null argument");
+    HH_ARG(((uintptr_t)d_rays & 15) == 0 && ((uintptr_t)d_hits & 3) == 0, "rays need 16-byte, hits 4-byte alignment");
+    if (n > kNoAllocMaxRays)
+        return fail(HH_ERR_LIMIT_EXCEEDED, "hh_trace_rays_noalloc: %u rays exceed the fixed scratch of %u rays", n,
+                    kNoAllocMaxRays);
+    hh_status st = require_committed(s);
+    if (st != HH_OK) return st;
+    hh_context ctx = s->ctx;
+    HH_HIP(hipSetDevice(ctx->device));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) return fail(HH_ERR_DEVICE, "hh_trace_rays_noalloc: no scratch address for device %d", ctx->device);
+    if (n == 0) return HH_OK;
+    uint32_t* spill = (uint32_t*)base;
+    float4* d_hit = (float4*)((char*)base + kNoAllocSpillBytes);
+    uint32_t* d_ids = (uint32_t*)((char*)base + kNoAllocSpillBytes + kNoAllocHitBytes);
+    LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
+    // The spill area of a k_trace_rays launch is strided by its thread count,
+    // cdiv(n, 256) * 256 <= kNoAllocMaxRays.
+    HH_HIP(launch_trace_rays(cfg, s->d_gpu, s->gpu, (const float4*)d_rays, n, flags, d_hit, d_ids, spill));
+    HH_HIP(launch_pack_hits(ctx->stream, d_hit, d_ids, n, d_hits));
     return HH_OK;
 }
 

@@ -1455,6 +1455,29 @@ hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, cons
     return hipGetLastError();
 }
 
+// One thread per hit: the 24-byte record {t, u, v, instance, geometry,
+// primitive} as six words.
+__global__ void __launch_bounds__(256) k_pack_hits(const float4* __restrict__ hit, const uint32_t* __restrict__ ids,
+                                                   uint32_t n, uint32_t* __restrict__ out) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 h = hit[i];
+    const bool miss = h.x < 0.0f;
+    uint32_t* o = out + 6 * (size_t)i;
+    o[0] = __float_as_uint(h.x);
+    o[1] = miss ? 0u : __float_as_uint(h.y);
+    o[2] = miss ? 0u : __float_as_uint(h.z);
+    o[3] = miss ? 0xFFFFFFFFu : __float_as_uint(h.w);
+    o[4] = miss ? 0xFFFFFFFFu : ids[2 * i];
+    o[5] = miss ? 0xFFFFFFFFu : ids[2 * i + 1];
+}
+
+hipError_t launch_pack_hits(hipStream_t stream, const float4* d_hit, const uint32_t* d_ids, uint32_t n, void* d_out) {
+    if (n == 0) return hipSuccess;
+    k_pack_hits<<<cdiv(n, 256), 256, 0, stream>>>(d_hit, d_ids, n, (uint32_t*)d_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
                                uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_ids, uint32_t* spill) {
     uint32_t n = w * h;

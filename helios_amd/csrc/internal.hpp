@@ -171,6 +171,12 @@ hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const Sc
                         Wavefront& wf, bool megakernel);
 hipError_t launch_trace_rays(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const float4* d_rays,
                              uint32_t n, uint32_t flags, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
+// Converts the raw hits of launch_trace_rays (t,u,v,bits(instance) + geometry,
+// primitive) into 24-byte hh_hit records {t, u, v, instance, geometry,
+// primitive} in device memory, misses as hh_trace_rays reports them
+// (u = v = 0, every index 0xFFFFFFFF).
+hipError_t launch_pack_hits(hipStream_t stream, const float4* d_hit, const uint32_t* d_hit_ids, uint32_t n,
+                            void* d_out);
 hipError_t launch_primary_hits(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,
                                uint32_t w, uint32_t h, uint32_t frame, float4* d_hit, uint32_t* d_hit_ids, uint32_t* spill);
 hipError_t launch_output_buffer(const LaunchCfg& cfg, const SceneGPU* d_scene, const SceneGPU& h_scene, const ViewGPU* d_view,

@@ -83,6 +83,7 @@ _SIGS = {
     "hh_context_autotune": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(abi.Tuning),
                             C.POINTER(C.c_double)],
     "hh_trace_rays": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
+    "hh_trace_rays_noalloc": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_primary_hits": [C.c_void_p, C.POINTER(abi.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p],
     "hh_sky_fit": [C.POINTER(abi.SkyParams), C.c_void_p, C.c_void_p],
     "hh_context_set_profiling": [C.c_void_p, C.c_uint32],
@@ -308,6 +309,12 @@ class Scene:
         out = np.zeros(len(rays), abi.HIT_DTYPE)
         check(lib().hh_trace_rays(self.h, ptr(rays), len(rays), flags, ptr(out)))
         return out
+
+    def trace_rays_device(self, d_rays, n, flags, d_hits):
+        """hh_trace_rays_noalloc on device buffers (addresses as integers): n
+        hh_ray records in, n hh_hit records out, asynchronous on the context's
+        stream; nothing is allocated."""
+        check(lib().hh_trace_rays_noalloc(self.h, C.c_void_p(d_rays), n, flags, C.c_void_p(d_hits)))
 
     def primary_hits(self, camera, width, height, frame=0):
         out = np.zeros((height, width), abi.HIT_DTYPE)

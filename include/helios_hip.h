@@ -482,6 +482,18 @@ hh_status hh_render_batch(hh_context ctx, const hh_view* views, uint32_t num_vie
  * semantics), used by parity tests.
  * ------------------------------------------------------------------------- */
 hh_status hh_trace_rays(hh_scene scene, const hh_ray* rays, uint32_t n, uint32_t flags, hh_hit* out_hits);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). hh_trace_rays on caller-owned DEVICE buffers: d_rays holds
+ * n hh_ray records (16-byte aligned), d_hits receives n hh_hit records, with
+ * the same results as hh_trace_rays. The traversal stacks and raw hits live in
+ * a fixed per-GPU scratch array inside the library (about 14.4 MB, held by each
+ * GPU it runs on until the process ends); nothing is allocated or freed.
+ * n <= HH_TRACE_NOALLOC_MAX_RAYS, larger n returns HH_ERR_LIMIT_EXCEEDED.
+ * Asynchronous on the context's stream (hh_sync waits); the scratch is shared
+ * by every call on one GPU, so calls that overlap on one GPU race. */
+#define HH_TRACE_NOALLOC_MAX_RAYS 32768u
+hh_status hh_trace_rays_noalloc(hh_scene scene, const hh_ray* d_rays, uint32_t n, uint32_t flags, hh_hit* d_hits);
 /* Primary rays of frame `num_frames` for every pixel, traced with flags 0. */
 hh_status hh_primary_hits(hh_scene scene, const hh_camera* cam, uint32_t width, uint32_t height,
                           uint32_t num_frames, hh_hit* out_hits);
