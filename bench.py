@@ -59,9 +59,10 @@ def parse():
     p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
     p.add_argument("--bvh-builder", type=int, default=1,
                    help="0 LBVH on the GPU, 1 binned SAH on the host, 2 PLOC clustering on the GPU")
-    p.add_argument("--continuation-depth", type=int, default=0,
+    p.add_argument("--continuation-depth", type=int, default=-1,
                    help="bounces traced as wavefront launches before one continuation launch finishes every path "
-                        "(0 = every bounce as wavefront launches)")
+                        "(0 = every bounce as wavefront launches; -1 = 4 when a pass holds fewer than 64 frames, "
+                        "else 0)")
     p.add_argument("--megakernel", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,6 +107,17 @@ def cpu_baseline(desc, budget_s):
                 gsample_bounces_per_s=ext / t / 1e9)
 
 
+def continuation_depth(requested, chunk):
+    """-1 picks the measured best for the pass size: with fewer than 64 frames
+    in flight the deep-bounce launches are small and tail-bound, and finishing
+    every path after 4 wavefront bounces in one continuation launch is faster
+    (20 frames: 1.241 -> 1.196 ms per frame); from 64 frames the wavefront
+    launches are full and continuing is not faster (256: 1.073 vs 1.081)."""
+    if requested >= 0:
+        return requested
+    return 4 if chunk < 64 else 0
+
+
 def single_frame(it, scene, desc, surf, pixels, calls=20):
     """The reference's protocol: one progressive frame per render call
     (path_integrator.cpp:48-84), on the single-frame integrator form (camera
@@ -124,6 +136,88 @@ def single_frame(it, scene, desc, surf, pixels, calls=20):
     return {"ms_per_frame": round(dt / calls * 1e3, 4), "msamples_per_s": round(pixels * calls / dt / 1e6, 1),
             "gsample_bounces_per_s": round(ext * calls / dt / 1e9, 4), "calls": calls,
             "integrator": "camera-ray wavefront + continuation megakernel, 1 frame per call"}
+
+
+def roofline(ctx, it, scene, desc, surf, chunk, node_bytes, a):
+    """Roofline of the closest-hit traversal kernels (k_extend_packet at depth
+    0, k_extend after), on one pass of `chunk` frames: the launch shape of the
+    timed run.
+
+    Pass 1 (HIP events on the launch stream) gives the extension launches'
+    time and the rays they traced. Pass 2 (the counting traversal variant,
+    every bounce as per-lane wavefront launches) gives node, triangle and
+    instance fetches per ray.
+
+    achieved = algorithmic HBM bytes of one launch / its average time, where
+    the algorithmic bytes are the launch's queue traffic (56 B per ray: ray
+    in 32, slot 4, hit record out 16 + 4) plus one read of the whole
+    acceleration structure (every BLAS/TLAS node, leaf triangle and instance
+    record once): what the launch must move from HBM with ideal caching.
+    traffic = HBM bytes per launch measured by FETCH_SIZE / WRITE_SIZE at the
+    same shape (profiles/extend_traffic.json), scaled by the copy-kernel
+    calibration. The per-lane fetch bytes (every lane's node, triangle and
+    instance fetches) are served mostly by L1/L2 and are reported beside it,
+    as is the issue bound the kernel actually runs into
+    (profiles/extend_issue.json)."""
+    ctx.set_profiling(timing=True)
+    ctx.reset_profile()
+    it.render(scene, desc.camera, surf, frames=chunk)
+    prof = ctx.read_profile()
+    ctx.set_profiling(timing=True, counting=True)
+    ctx.reset_profile()
+    it.render(scene, desc.camera, surf, frames=chunk)
+    cprof = ctx.read_profile()
+    ctx.set_profiling()
+    cnt = cprof["counts"]
+    crays = max(cprof["extend"]["extension_rays"], 1)
+    csrays = max(cprof["shadow"]["shadow_rays"], 1)
+    ext = prof["extend"]
+    launches = max(ext["launches"], 1)
+    avg_ms = ext["ms"] / launches
+    rays_per_launch = ext["extension_rays"] / launches
+    bs = scene.bvh_stats()
+    scene_bytes = ((bs["blas_nodes"] + bs["tlas_nodes"]) * node_bytes + bs["triangles"] * TRI_BYTES +
+                   len(desc.instances) * INST_BYTES)
+    alg = rays_per_launch * RAY_QUEUE_BYTES + scene_bytes
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    per_ray = {"nodes": cnt["ext_nodes"] / crays, "tris": cnt["ext_tris"] / crays, "insts": cnt["ext_insts"] / crays}
+    lane_bytes = rays_per_launch * (per_ray["nodes"] * node_bytes + per_ray["tris"] * TRI_BYTES +
+                                    per_ray["insts"] * INST_BYTES + RAY_QUEUE_BYTES)
+    shape = {"frames_in_flight": chunk, "continuation_depth": a.continuation_depth, "bvh_builder": a.bvh_builder}
+    traffic, l2, issue = None, None, None
+    tfile, ifile = ROOT / "profiles" / "extend_traffic.json", ROOT / "profiles" / "extend_issue.json"
+    if tfile.exists():
+        tj = json.loads(tfile.read_text())
+        if tj.get("shape") == shape:
+            traffic, l2 = tj.get("hbm_bytes_per_launch"), tj.get("l2")
+    if ifile.exists():
+        ij = json.loads(ifile.read_text())
+        if ij.get("shape") == shape:
+            issue = {k: ij["kernels"][k] for k in ij.get("kernels", {})}
+    r = {
+        "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
+        "kernel": "k_extend_packet (depth 0) + k_extend (depths 1-%d): closest-hit traversal" % (
+            (a.continuation_depth or a.bounces) - 1),
+        "launches_per_pass": ext["launches"], "avg_launch_ms": round(avg_ms, 4),
+        "rays_per_launch": round(rays_per_launch),
+        "alg_bytes_per_launch": round(alg),
+        "alg_bytes": "56 B queue traffic per ray + one read of the acceleration structure (%d B)" % scene_bytes,
+        "hbm_gbps": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "lane_fetch_bytes_per_launch": round(lane_bytes),
+        "lane_fetch_gbps": round(lane_bytes / (avg_ms * 1e-3) / 1e9, 1),
+        "l2_read_gbps": l2.get("l2_read_gbps") if l2 else None,
+        "l1_hit_rate": l2.get("l1_hit_rate") if l2 else None,
+        "l2_hit_rate": l2.get("l2_hit_rate") if l2 else None,
+        "issue": issue,
+        "per_ray": {k: round(v, 2) for k, v in per_ray.items()},
+        "shadow_per_ray": {"nodes": round(cnt["shadow_nodes"] / csrays, 2),
+                           "tris": round(cnt["shadow_tris"] / csrays, 2),
+                           "insts": round(cnt["shadow_insts"] / csrays, 2)},
+        "shape": shape,
+    }
+    kms = {k: round(v["ms"] / chunk, 4) for k, v in prof.items() if k != "counts" and v["launches"]}
+    return r, kms
 
 
 def relmse(x, r, eps=0.01):
@@ -411,6 +505,10 @@ def main():
 
     desc = scenegen.icosphere_field(seed=0, width=a.width, height=a.height, max_ray_bounces=a.bounces)
     ctx = runtime.Context(0)
+    # The timed call traces `steps` frames in passes of `chunk` frames (plus a
+    # remainder pass); --probe traces one pass of frames_in_flight frames.
+    chunk = a.frames_in_flight if a.probe else max(1, min(a.steps, a.frames_in_flight))
+    a.continuation_depth = continuation_depth(a.continuation_depth, chunk)
     ctx.set_tuning(bvh_width=a.bvh_width, extend_block=a.block, persistent_blocks_per_cu=a.blocks_per_cu,
                    frames_in_flight=a.frames_in_flight, bvh_compressed=a.bvh_compressed,
                    refill_lanes=a.refill_lanes, traversal=a.traversal, bvh_builder=a.bvh_builder,
@@ -448,15 +546,15 @@ def main():
         ctx.sync()
         gbps = ctx.bandwidth_probe(1 << 30, 1)
         print(json.dumps({"probe": f"{a.frames_in_flight} frames (one chunk) + 1 GiB copy", "copy_gbps": gbps,
-                          "rays": it.ray_stats()}))
+                          "rays": it.ray_stats(),
+                          "shape": {"frames_in_flight": chunk, "continuation_depth": a.continuation_depth,
+                                    "bvh_builder": a.bvh_builder}}))
         return 0
 
-    # The timed call traces `steps` frames in passes of `chunk` frames (plus a
-    # remainder pass). Warm-up first renders the W requested frames, then the
-    # timed call's own pass sequence once, so the path state is already sized
-    # for the largest pass and every pass's hipGraph is captured and
-    # instantiated before the clock starts.
-    chunk = max(1, min(a.steps, a.frames_in_flight))
+    # Warm-up first renders the W requested frames, then the timed call's own
+    # pass sequence once, so the path state is already sized for the largest
+    # pass and every pass's hipGraph is captured and instantiated before the
+    # clock starts.
     rem = a.steps % chunk
     it.render(scene, desc.camera, surf, frames=a.warmup)
     it.render(scene, desc.camera, surf, frames=chunk)
@@ -505,56 +603,7 @@ def main():
     }
 
     if not a.no_roofline and not a.megakernel:
-        # (1) kernel durations from HIP events on the launch stream, (2) fetch
-        # counts from the counting traversal variant on the same frames.
-        # Both passes trace one pass of `chunk` frames, the launch shape of
-        # the timed run, so the per-launch figures describe the same
-        # dispatches rocprofv3 averages.
-        ctx.set_profiling(timing=True)
-        ctx.reset_profile()
-        it.render(scene, desc.camera, surf, frames=chunk)
-        prof = ctx.read_profile()
-        ctx.set_profiling(timing=True, counting=True)
-        ctx.reset_profile()
-        it.render(scene, desc.camera, surf, frames=chunk)
-        cprof = ctx.read_profile()
-        cnt = cprof["counts"]
-        rstats = it.ray_stats()
-        rays, srays = rstats["extension_rays"], max(rstats["shadow_rays"], 1)
-        ctx.set_profiling()
-        ext = prof["extend"]
-        alg_bytes = (cnt["ext_nodes"] * node_bytes + cnt["ext_tris"] * TRI_BYTES +
-                     cnt["ext_insts"] * INST_BYTES + rays * RAY_QUEUE_BYTES)
-        per_launch = alg_bytes / max(cprof["extend"]["launches"], 1)
-        avg_ms = ext["ms"] / max(ext["launches"], 1)
-        achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        traffic, l2 = None, None
-        tfile = ROOT / "profiles" / "extend_traffic.json"
-        if tfile.exists():
-            tj = json.loads(tfile.read_text())
-            traffic = tj.get("hbm_bytes_per_launch")
-            l2 = tj.get("l2")
-        out["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
-            "kernel": "k_extend (closest-hit traversal)",
-            "note": ("achieved counts every lane's node, triangle and instance fetches (algorithmic bytes); lanes of "
-                     "a coherent wave fetch the same lines and the scene fits the 256 MB Infinity Cache, so it can "
-                     "exceed the HBM peak; hbm_gbps is the measured HBM traffic rate"),
-            "hbm_gbps": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
-            "l2_read_gbps": l2.get("l2_read_gbps") if l2 else None,
-            "l1_hit_rate": l2.get("l1_hit_rate") if l2 else None,
-            "l2_hit_rate": l2.get("l2_hit_rate") if l2 else None,
-            "alg_bytes_per_launch": round(per_launch),
-            "avg_launch_ms": round(avg_ms, 4),
-            "per_ray": {"nodes": round(cnt["ext_nodes"] / rays, 2), "tris": round(cnt["ext_tris"] / rays, 2),
-                        "insts": round(cnt["ext_insts"] / rays, 2)},
-            "shadow_per_ray": {"nodes": round(cnt["shadow_nodes"] / srays, 2),
-                               "tris": round(cnt["shadow_tris"] / srays, 2),
-                               "insts": round(cnt["shadow_insts"] / srays, 2)},
-        }
-        out["kernel_ms_per_frame"] = {k: round(v["ms"] / chunk, 4) for k, v in prof.items()
-                                      if k != "counts" and v["launches"]}
+        out["roofline"], out["kernel_ms_per_frame"] = roofline(ctx, it, scene, desc, surf, chunk, node_bytes, a)
 
     if not a.megakernel:
         out["single_frame"] = single_frame(it, scene, desc, surf, pixels)

@@ -404,6 +404,7 @@ hh_status hh_context_destroy(hh_context ctx) {
     if (ctx->dbg) (void)hipFree(ctx->dbg);
     if (ctx->d_coef) (void)hipFree(ctx->d_coef);
     if (ctx->prof.d_counts) (void)hipFree(ctx->prof.d_counts);
+    if (ctx->prof.d_snaps) (void)hipFree(ctx->prof.d_snaps);
     for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -452,6 +453,8 @@ hh_status hh_context_set_profiling(hh_context ctx, uint32_t flags) {
     HH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->prof.timing = (flags & 1u) != 0;
     ctx->prof.count = (flags & 2u) != 0;
+    if (ctx->prof.timing && !ctx->prof.d_snaps)
+        HH_HIP(hipMalloc(&ctx->prof.d_snaps, size_t(Prof::kSnapCap) * 4 * sizeof(unsigned long long)));
     if (ctx->prof.count && !ctx->prof.d_counts) {
         HH_HIP(hipMalloc(&ctx->prof.d_counts, 8 * sizeof(unsigned long long)));
         HH_HIP(hipMemset(ctx->prof.d_counts, 0, 8 * sizeof(unsigned long long)));
@@ -463,14 +466,23 @@ hh_status hh_context_read_profile(hh_context ctx, double ms[6], uint64_t launche
     HH_ARG(ctx && ms && launches && counts, "null argument");
     HH_HIP(hipStreamSynchronize(ctx->stream));
     Prof& p = ctx->prof;
+    std::vector<unsigned long long> snaps(size_t(p.snaps_used) * 4);
+    if (p.snaps_used)
+        HH_HIP(hipMemcpy(snaps.data(), p.d_snaps, snaps.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (const Prof::Rec& r : p.recs) {
         float t = 0.f;
         HH_HIP(hipEventElapsedTime(&t, r.a, r.b));
         p.ms[r.kind] += t;
         p.launches[r.kind]++;
+        if (r.snap >= 0) {
+            const unsigned long long* q = &snaps[4 * size_t(r.snap)];
+            p.rays[r.kind][0] += q[2] - q[0];
+            p.rays[r.kind][1] += q[3] - q[1];
+        }
     }
     p.recs.clear();
     p.used = 0;
+    p.snaps_used = 0;
     for (int k = 0; k < PK_COUNT; k++) {
         ms[k] = p.ms[k];
         launches[k] = p.launches[k];
@@ -481,15 +493,26 @@ hh_status hh_context_read_profile(hh_context ctx, double ms[6], uint64_t launche
     return HH_OK;
 }
 
+hh_status hh_context_profile_rays(hh_context ctx, uint64_t extension[6], uint64_t shadow[6]) {
+    HH_ARG(ctx && extension && shadow, "null argument");
+    for (int k = 0; k < PK_COUNT; k++) {
+        extension[k] = ctx->prof.rays[k][0];
+        shadow[k] = ctx->prof.rays[k][1];
+    }
+    return HH_OK;
+}
+
 hh_status hh_context_reset_profile(hh_context ctx) {
     HH_ARG(ctx, "null context");
     HH_HIP(hipStreamSynchronize(ctx->stream));
     Prof& p = ctx->prof;
     p.recs.clear();
     p.used = 0;
+    p.snaps_used = 0;
     for (int k = 0; k < PK_COUNT; k++) {
         p.ms[k] = 0;
         p.launches[k] = 0;
+        p.rays[k][0] = p.rays[k][1] = 0;
     }
     if (p.d_counts) HH_HIP(hipMemset(p.d_counts, 0, 8 * sizeof(unsigned long long)));
     return HH_OK;

@@ -1303,8 +1303,13 @@ struct ProfScope {
     int kind;
     hipStream_t st;
     hipEvent_t a = nullptr;
+    int snap = -1;
     ProfScope(Prof* p_, int k, hipStream_t s) : p(p_), kind(k), st(s) {
         if (p && p->timing) {
+            if (p->d_snaps && p->stats && p->snaps_used < Prof::kSnapCap) {
+                snap = p->snaps_used++;
+                (void)hipMemcpyAsync(p->d_snaps + 4 * snap, p->stats, 16, hipMemcpyDeviceToDevice, st);
+            }
             a = p->ev();
             if (a) (void)hipEventRecord(a, st);
         }
@@ -1314,8 +1319,9 @@ struct ProfScope {
             hipEvent_t b = p->ev();
             if (b) {
                 (void)hipEventRecord(b, st);
-                p->recs.push_back({kind, a, b});
+                p->recs.push_back({kind, a, b, snap});
             }
+            if (snap >= 0) (void)hipMemcpyAsync(p->d_snaps + 4 * snap + 2, p->stats, 16, hipMemcpyDeviceToDevice, st);
         }
     }
 };
@@ -1328,6 +1334,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     hipStream_t st = cfg.stream;
     Prof* pf = cfg.prof;
     unsigned long long* tc = pf ? pf->d_counts : nullptr;
+    if (pf) pf->stats = wf.stats;
     const uint32_t grid = frame_grid(total, cfg.num_cu, cfg.tune.blocks_per_cu, BS);
     const uint32_t refill = cfg.tune.refill_lanes;
     // A wave of camera rays holds jittered samples of one or two pixels, which
@@ -1393,7 +1400,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     // launches. cont = 0: every bounce as wavefront launches.
     // (the counting variant traces every bounce as wavefront launches: it
     // counts the fetches of the per-lane traversal kernels)
-    const uint32_t cont = COUNT ? 0u : (cfg.tune.continuation_depth ? cfg.tune.continuation_depth : (mega ? 1u : 0u));
+    const uint32_t cont = COUNT ? 0u : (mega ? 1u : cfg.tune.continuation_depth);
     if (cont > 0 && cont < max_bounces) {
         for (uint32_t d = 0; d < cont; d++) bounce(d);
         const int q = (int)(cont & 1u);  // bounce(cont - 1) queued its extension rays there

@@ -142,6 +142,7 @@ struct Prof {
     struct Rec {
         int kind;
         hipEvent_t a, b;
+        int snap;   // index into d_snaps, -1 = none
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
@@ -149,6 +150,14 @@ struct Prof {
     double ms[PK_COUNT] = {0};
     unsigned long long launches[PK_COUNT] = {0};
     unsigned long long* d_counts = nullptr;  // [ext nodes, ext tris, ext insts, shadow nodes, shadow tris, shadow insts]
+    // Rays traced per launch kind: the frame's ray statistics (extension,
+    // shadow) copied before and after every timed launch (4 words per launch,
+    // kSnapCap launches between two reads), attributed to the launch's kind.
+    static constexpr int kSnapCap = 4096;
+    unsigned long long* d_snaps = nullptr;
+    int snaps_used = 0;
+    const unsigned long long* stats = nullptr;   // set by the frame being profiled
+    unsigned long long rays[PK_COUNT][2] = {};
     hipEvent_t ev() {
         if (used == pool.size()) {
             hipEvent_t e;

@@ -5,6 +5,8 @@
          --kernel k_extend --cmd "<command>" --out profiles/extend_traffic.json
   python -m helios_amd.profiling mfma <kernel_stats.csv> <counter_collection.csv> --cmd "<commands>"
          --out profiles/<name>.md
+  python -m helios_amd.profiling issue <counter_collection.csv> --cmd "<command>" --shape '<json>'
+         --out profiles/extend_issue.json
 
 `traffic` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) by what the same
 counters read for the plain copy kernel k_copy_probe, which moves a known
@@ -49,6 +51,29 @@ def stats_md(csv_path, cmd, top=12):
         lines.append(f"| {short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                      f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
                      f"{float(r['Percentage']):.2f} |")
+    return "\n".join(lines) + "\n"
+
+
+def passes_md(trace_csv, max_rows=12):
+    """Extension launches (k_extend_packet + k_extend, not the counting
+    instantiations) grouped by wavefront pass: a pass starts at each
+    k_generate dispatch. Lists launches and average duration per pass, in
+    dispatch order, so the timed pass can be matched with the bench's
+    HIP-event figure."""
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    passes, cur = [], None
+    for r in rows:
+        n = r["Kernel_Name"]
+        if "k_generate<" in n:
+            cur = {"gen": short(n), "ext": []}
+            passes.append(cur)
+        elif cur is not None and ("k_extend_packet<" in n or ("k_extend<" in n and "false>" in n)):
+            cur["ext"].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    lines = ["", "Extension launches per wavefront pass (a pass starts at each k_generate dispatch):", "",
+             "| pass | generate | extension launches | avg us |", "|---|---|---|---|"]
+    for i, pz in enumerate(passes[:max_rows]):
+        if pz["ext"]:
+            lines.append(f"| {i + 1} | {pz['gen']} | {len(pz['ext'])} | {sum(pz['ext']) / len(pz['ext']) / 1e3:.1f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -147,6 +172,45 @@ def traffic(fetch_csv, write_csv, kernel, cmd):
     }
 
 
+ISSUE_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_BUSY_CYCLES",
+                  "GRBM_GUI_ACTIVE")
+
+
+def issue(pmc_csv, cmd, simds=1024, xcds=8, cycles_per_valu=4):
+    """Issue-side limits of the traversal and shading kernels, per dispatch:
+
+      valu_issue_frac  = SQ_INSTS_VALU x 4 cycles (a wave64 VALU instruction
+                         occupies its SIMD for 4 cycles)
+                         / (SIMDs x GRBM_GUI_ACTIVE / XCDs): the share of the
+                         chip's VALU issue cycles the kernel's instructions use
+      lane_utilization = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU):
+                         active lanes per issued VALU instruction
+      valu_insts_per_wave = SQ_INSTS_VALU / SQ_WAVES
+    GRBM_GUI_ACTIVE is summed over the XCDs."""
+    agg = per_dispatch_multi(pmc_csv)
+    out = {}
+    for name, c in agg.items():
+        base = name.split("<")[0].split("::")[-1]
+        if base not in ("k_extend", "k_extend_packet", "k_shadow", "k_shade", "k_megakernel") or "true" in name:
+            continue
+        if any(k not in c for k in ISSUE_COUNTERS):
+            continue
+        avg = {k: sum(v) / len(v) for k, v in c.items()}
+        cyc = avg["GRBM_GUI_ACTIVE"] / xcds
+        ent = {
+            "dispatches": len(c["SQ_WAVES"]),
+            "valu_issue_frac": round(avg["SQ_INSTS_VALU"] * cycles_per_valu / (simds * cyc), 4) if cyc else None,
+            "lane_utilization": round(avg["SQ_THREAD_CYCLES_VALU"] / (64 * avg["SQ_ACTIVE_INST_VALU"]), 4)
+            if avg["SQ_ACTIVE_INST_VALU"] else None,
+            "valu_insts_per_wave": round(avg["SQ_INSTS_VALU"] / avg["SQ_WAVES"], 1) if avg["SQ_WAVES"] else None,
+            "raw": {k: round(avg[k], 1) for k in ISSUE_COUNTERS},
+        }
+        prev = out.get(base)
+        if prev is None or prev["raw"]["GRBM_GUI_ACTIVE"] < ent["raw"]["GRBM_GUI_ACTIVE"]:
+            out[base] = ent   # the instantiation that ran longest (the timed one)
+    return {"command": cmd, "formula": " ".join(issue.__doc__.split()), "kernels": out}
+
+
 def main():
     p = argparse.ArgumentParser()
     sub = p.add_subparsers(dest="what", required=True)
@@ -154,6 +218,8 @@ def main():
     s.add_argument("csv")
     s.add_argument("--cmd", required=True)
     s.add_argument("--title", default="")
+    s.add_argument("--trace", default=None, help="kernel_trace.csv of the same run: per-pass extension launches")
+    s.add_argument("--note", default="")
     s.add_argument("--out", required=True)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
@@ -164,6 +230,12 @@ def main():
     t.add_argument("--l2-csv", default=None, help="counter run with TCP_TCC_READ_REQ_sum, TCC_HIT_sum, ...")
     t.add_argument("--l2-cmd", default="")
     t.add_argument("--avg-ms", type=float, default=0.0, help="the kernel's average launch time")
+    t.add_argument("--shape", default=None, help="JSON launch shape the counters were taken at")
+    i = sub.add_parser("issue")
+    i.add_argument("pmc_csv")
+    i.add_argument("--cmd", required=True)
+    i.add_argument("--shape", default=None)
+    i.add_argument("--out", required=True)
     m = sub.add_parser("mfma")
     m.add_argument("stats_csv")
     m.add_argument("pmc_csv")
@@ -172,13 +244,24 @@ def main():
     a = p.parse_args()
     if a.what == "stats":
         text = (f"# {a.title}\n\n" if a.title else "") + stats_md(a.csv, a.cmd)
+        if a.trace:
+            text += passes_md(a.trace)
+        if a.note:
+            text += "\n" + a.note + "\n"
         Path(a.out).write_text(text)
     elif a.what == "mfma":
         Path(a.out).write_text(mfma_md(a.stats_csv, a.pmc_csv, a.cmd))
+    elif a.what == "issue":
+        res = issue(a.pmc_csv, a.cmd)
+        if a.shape:
+            res["shape"] = json.loads(a.shape)
+        Path(a.out).write_text(json.dumps(res, indent=1) + "\n")
     else:
         res = traffic(a.fetch_csv, a.write_csv, a.kernel, a.cmd)
         if a.l2_csv:
             res["l2"] = l2_traffic(a.l2_csv, a.kernel, a.avg_ms, a.l2_cmd)
+        if a.shape:
+            res["shape"] = json.loads(a.shape)
         Path(a.out).write_text(json.dumps(res, indent=1) + "\n")
     print("wrote", a.out)
 

@@ -89,6 +89,7 @@ _SIGS = {
     "hh_context_set_profiling": [C.c_void_p, C.c_uint32],
     "hh_context_read_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "hh_context_reset_profile": [C.c_void_p],
+    "hh_context_profile_rays": [C.c_void_p, C.c_void_p, C.c_void_p],
     "hh_bandwidth_probe": [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)],
     "hh_scene_load_gltf": [C.c_void_p, C.c_char_p, C.POINTER(abi.GltfInfo)],
     "hh_ibl_from_hdr": [C.c_char_p, C.c_uint32, C.c_void_p],
@@ -189,13 +190,18 @@ class Context:
         return out.value
 
     def read_profile(self):
-        """Per kernel kind: total ms and launches (HIP events on the context
-        stream), plus traversal fetch counts of the counting variant."""
+        """Per kernel kind: total ms, launches (HIP events on the context
+        stream) and the extension / shadow rays those launches traced, plus
+        traversal fetch counts of the counting variant."""
         ms = np.zeros(6, np.float64)
         n = np.zeros(6, np.uint64)
         c = np.zeros(6, np.uint64)
         check(lib().hh_context_read_profile(self.h, ptr(ms), ptr(n), ptr(c)))
-        out = {k: dict(ms=float(ms[i]), launches=int(n[i])) for i, k in enumerate(KERNEL_KINDS)}
+        er = np.zeros(6, np.uint64)
+        sr = np.zeros(6, np.uint64)
+        check(lib().hh_context_profile_rays(self.h, ptr(er), ptr(sr)))
+        out = {k: dict(ms=float(ms[i]), launches=int(n[i]), extension_rays=int(er[i]), shadow_rays=int(sr[i]))
+               for i, k in enumerate(KERNEL_KINDS)}
         out["counts"] = dict(ext_nodes=int(c[0]), ext_tris=int(c[1]), ext_insts=int(c[2]), shadow_nodes=int(c[3]),
                              shadow_tris=int(c[4]), shadow_insts=int(c[5]))
         return out

@@ -534,9 +534,9 @@ typedef struct hh_tuning {
     /* Paths still alive after this many bounces traced as wavefront launches
      * finish in one continuation launch, in which a lane whose path ends takes
      * the next queued path (deep bounces hold few rays, and a launch per bounce
-     * pays its latency tail each time). 0 = every bounce as wavefront launches
-     * (HH_FLAG_MEGAKERNEL then continues after the camera rays' bounce, i.e.
-     * 1); 1..64 otherwise. Results are identical. */
+     * pays its latency tail each time). 0 = every bounce as wavefront launches,
+     * 1..64 otherwise; HH_FLAG_MEGAKERNEL (the single-frame form) always
+     * continues after the camera rays' bounce, i.e. 1. Results are identical. */
     uint32_t continuation_depth;
 } hh_tuning;
 hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t);
@@ -568,6 +568,10 @@ hh_status hh_context_autotune(hh_context ctx, const hh_view* views, uint32_t num
 hh_status hh_context_set_profiling(hh_context ctx, uint32_t flags);
 hh_status hh_context_read_profile(hh_context ctx, double ms[6], uint64_t launches[6], uint64_t counts[6]);
 hh_status hh_context_reset_profile(hh_context ctx);
+/* With timing on: extension and shadow rays traced by the launches of each
+ * kernel kind since the last reset (as of the last hh_context_read_profile),
+ * so per-launch work matches the timed launches exactly. */
+hh_status hh_context_profile_rays(hh_context ctx, uint64_t extension[6], uint64_t shadow[6]);
 /* Streams `bytes` through a plain copy kernel `iters` times and returns the
  * achieved read+write GB/s: the calibration point for HBM byte counters. */
 hh_status hh_bandwidth_probe(hh_context ctx, uint64_t bytes, uint32_t iters, double* out_gbps);
