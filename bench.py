@@ -39,7 +39,7 @@ NODE_BYTES = {4: 128, 8: 256}
 TRI_BYTES = 48           # 3 x float4 leaf-ordered triangle
 INST_BYTES = 64          # InstTrav record (world->object 3x4 + BLAS root)
 RAY_QUEUE_BYTES = 32 + 4 + 16 + 4   # ray in (o, d), slot, hit record out
-BUILDERS = {0: "LBVH (GPU)", 1: "binned SAH (host)", 2: "PLOC clustering (GPU)"}
+BUILDERS = {0: "LBVH (GPU)", 1: "binned SAH (host)", 2: "PLOC clustering (GPU)", 3: "binned SAH (GPU)"}
 
 
 def parse():
@@ -57,8 +57,9 @@ def parse():
     p.add_argument("--frames-in-flight", type=int, default=256)
     p.add_argument("--refill-lanes", type=int, default=24)
     p.add_argument("--traversal", type=int, default=0, help="0 while-while, 1 if-if")
-    p.add_argument("--bvh-builder", type=int, default=1,
-                   help="0 LBVH on the GPU, 1 binned SAH on the host, 2 PLOC clustering on the GPU")
+    p.add_argument("--bvh-builder", type=int, default=3,
+                   help="0 LBVH on the GPU, 1 binned SAH on the host, 2 PLOC clustering on the GPU, "
+                        "3 binned SAH on the GPU (the host builder's tree)")
     p.add_argument("--continuation-depth", type=int, default=-1,
                    help="bounces traced as wavefront launches before one continuation launch finishes every path "
                         "(0 = every bounce as wavefront launches; -1 = 4 when a pass holds fewer than 64 frames, "

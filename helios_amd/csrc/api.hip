@@ -20,6 +20,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/helios_hip.h"
@@ -431,7 +432,7 @@ hh_status hh_context_set_tuning(hh_context ctx, const hh_tuning* t) {
     HH_ARG(t->frames_in_flight >= 1 && t->frames_in_flight <= 256, "frames_in_flight must be 1..256");
     HH_ARG(t->refill_lanes >= 1 && t->refill_lanes <= 64, "refill_lanes must be 1..64");
     HH_ARG(t->traversal <= 1, "traversal must be 0 (while-while) or 1 (if-if)");
-    HH_ARG(t->bvh_builder <= 2, "bvh_builder must be 0 (LBVH), 1 (binned SAH on the host) or 2 (PLOC)");
+    HH_ARG(t->bvh_builder <= 3, "bvh_builder must be 0 (LBVH), 1 (binned SAH on the host), 2 (PLOC) or 3 (binned SAH on the GPU)");
     HH_ARG(t->continuation_depth <= 64, "continuation_depth must be 0..64");
     ctx->tune.bvh_builder = t->bvh_builder;
     ctx->tune.continuation_depth = t->continuation_depth;
@@ -643,6 +644,10 @@ hh_status hh_scene_add_mesh(hh_scene s, const hh_vertex* v, uint32_t nv, const u
             return fail(HH_ERR_INVALID_ARG, "submesh %u index range invalid", i);
     }
     for (uint32_t i = 0; i < nm; i++) HH_ARG(mats[i] < s->materials.size(), "mesh references an unknown material");
+    // positions feed the BVH builders' bounds and bins, which need finite values
+    for (uint32_t i = 0; i < nv; i++)
+        if (!(std::isfinite(v[i].position[0]) && std::isfinite(v[i].position[1]) && std::isfinite(v[i].position[2])))
+            return fail(HH_ERR_INVALID_ARG, "vertex %u has a non-finite position", i);
     MeshHost m;
     m.v.assign(v, v + nv);
     m.idx.assign(idx, idx + ni);
@@ -893,66 +898,101 @@ hh_status hh_scene_commit(hh_scene s) {
     std::vector<uint32_t> blas_root(global_meshes.size());
     std::vector<float> blas_box(global_meshes.size() * 6);
     uint32_t node_base = 0, tri_base = 0;
-    float4* d_soup = nullptr;
-    float4* d_shade_soup = nullptr;
-    size_t soup_cap = 0;
-    for (uint32_t mt : mesh_tris) soup_cap = std::max<size_t>(soup_cap, mt);
-    HH_HIP(hipMalloc(&d_soup, soup_cap * 3 * sizeof(float4)));
-    if (hipMalloc(&d_shade_soup, soup_cap * 4 * sizeof(float4)) != hipSuccess) {
-        (void)hipFree(d_soup);
-        return fail(HH_ERR_OUT_OF_MEMORY, "shading records");
-    }
-    // Per-triangle shading record (4 float4, leaf order like the positions):
-    // (u0 v0 u1 v1) (u2 v2 n0.x n0.y) (n0.z n1.xyz) (n2.xyz 0): the vertex
-    // attributes the shading and the alpha test interpolate, in one line.
-    std::vector<float4> soup, shade_soup;
-    hh_status st = HH_OK;
-    for (size_t g = 0; g < global_meshes.size() && st == HH_OK; g++) {
+    // Per-triangle positions (3 float4: xyz + primitive index / geometry index
+    // and opaque bit in w) and shading records (4 float4, leaf order like the
+    // positions): (u0 v0 u1 v1) (u2 v2 n0.x n0.y) (n0.z n1.xyz) (n2.xyz 0), the
+    // vertex attributes the shading and the alpha test interpolate, in one
+    // line. Mesh g owns triangles tri_off[g]..; an empty mesh one inert
+    // triangle. Filled by host threads, one mesh at a time per thread.
+    const size_t G = global_meshes.size();
+    std::vector<uint32_t> tri_off(G + 1, 0);
+    for (size_t g = 0; g < G; g++) tri_off[g + 1] = tri_off[g] + mesh_tris[g];
+    std::vector<float4> soup(ntri * 3), shade_soup(ntri * 4);
+    auto fill_mesh = [&](size_t g) {
         const MeshHost& me = s->meshes[global_meshes[g]];
-        soup.clear();
-        shade_soup.clear();
+        float4* ps = &soup[3 * size_t(tri_off[g])];
+        float4* ss = &shade_soup[4 * size_t(tri_off[g])];
+        size_t t = 0;
         for (uint32_t gi = 0; gi < me.sm.size(); gi++) {
             const hh_submesh& sm = me.sm[gi];
             const hh_material_desc& m = s->materials[me.mats[sm.mat_idx]];
             uint32_t opaque = (m.type == HH_MATERIAL_OPAQUE && !m.alpha_test) ? 0x80000000u : 0u;  // mesh.cpp:75-76
-            for (uint32_t p = 0; p < sm.index_count / 3; p++) {
+            for (uint32_t p = 0; p < sm.index_count / 3; p++, t++) {
                 const hh_vertex* v3[3];
                 for (int k = 0; k < 3; k++) v3[k] = &me.v[me.idx[sm.base_index + 3 * p + k]];
                 uint32_t w0 = p, w1 = gi | opaque;
                 float fw0, fw1;
                 std::memcpy(&fw0, &w0, 4);
                 std::memcpy(&fw1, &w1, 4);
-                soup.push_back(make_float4(v3[0]->position[0], v3[0]->position[1], v3[0]->position[2], fw0));
-                soup.push_back(make_float4(v3[1]->position[0], v3[1]->position[1], v3[1]->position[2], fw1));
-                soup.push_back(make_float4(v3[2]->position[0], v3[2]->position[1], v3[2]->position[2], 0.f));
-                shade_soup.push_back(make_float4(v3[0]->tex_coord[0], v3[0]->tex_coord[1], v3[1]->tex_coord[0], v3[1]->tex_coord[1]));
-                shade_soup.push_back(make_float4(v3[2]->tex_coord[0], v3[2]->tex_coord[1], v3[0]->normal[0], v3[0]->normal[1]));
-                shade_soup.push_back(make_float4(v3[0]->normal[2], v3[1]->normal[0], v3[1]->normal[1], v3[1]->normal[2]));
-                shade_soup.push_back(make_float4(v3[2]->normal[0], v3[2]->normal[1], v3[2]->normal[2], 0.f));
+                ps[3 * t] = make_float4(v3[0]->position[0], v3[0]->position[1], v3[0]->position[2], fw0);
+                ps[3 * t + 1] = make_float4(v3[1]->position[0], v3[1]->position[1], v3[1]->position[2], fw1);
+                ps[3 * t + 2] = make_float4(v3[2]->position[0], v3[2]->position[1], v3[2]->position[2], 0.f);
+                ss[4 * t] = make_float4(v3[0]->tex_coord[0], v3[0]->tex_coord[1], v3[1]->tex_coord[0], v3[1]->tex_coord[1]);
+                ss[4 * t + 1] = make_float4(v3[2]->tex_coord[0], v3[2]->tex_coord[1], v3[0]->normal[0], v3[0]->normal[1]);
+                ss[4 * t + 2] = make_float4(v3[0]->normal[2], v3[1]->normal[0], v3[1]->normal[1], v3[1]->normal[2]);
+                ss[4 * t + 3] = make_float4(v3[2]->normal[0], v3[2]->normal[1], v3[2]->normal[2], 0.f);
             }
         }
-        if (soup.empty()) {
-            for (int k = 0; k < 3; k++) soup.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
-            for (int k = 0; k < 4; k++) shade_soup.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+        if (t == 0) {
+            for (int k = 0; k < 3; k++) ps[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = 0; k < 4; k++) ss[k] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        uint32_t n = (uint32_t)(soup.size() / 3);
-        hipError_t e = hipMemcpyAsync(d_soup, soup.data(), soup.size() * sizeof(float4), hipMemcpyHostToDevice, ctx->stream);
+    };
+    {
+        const size_t nth = std::min<size_t>(G, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < nth; w++)
+            pool.emplace_back([&, w] {
+                for (size_t g = w; g < G; g += nth) fill_mesh(g);
+            });
+        for (size_t g = 0; g < G; g += nth) fill_mesh(g);
+        for (auto& th : pool) th.join();
+    }
+    float4* d_soup = nullptr;
+    float4* d_shade_soup = nullptr;
+    HH_HIP(hipMalloc(&d_soup, soup.size() * sizeof(float4)));
+    if (hipMalloc(&d_shade_soup, shade_soup.size() * sizeof(float4)) != hipSuccess) {
+        (void)hipFree(d_soup);
+        return fail(HH_ERR_OUT_OF_MEMORY, "shading records");
+    }
+    hipError_t e = hipSuccess;
+    if (ntri > 0) {
+        e = hipMemcpyAsync(d_soup, soup.data(), soup.size() * sizeof(float4), hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_shade_soup, shade_soup.data(), shade_soup.size() * sizeof(float4), hipMemcpyHostToDevice,
                                ctx->stream);
-        uint32_t nn = 0;
-        if (e == hipSuccess)
-            e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup, d_tris, d_shade_soup, d_shade,
-                           ctx->tune.bvh_builder == 1 ? soup.data() : nullptr, tri_base, d_nodes,
-                           node_base, ctx->bs, &nn,
-                           &blas_box[g * 6], ctx->tune.bvh_builder == 2);
-        if (e != hipSuccess) {
-            st = hip_fail(e, "BLAS build");
-            break;
+    }
+    hh_status st = HH_OK;
+    if (e != hipSuccess) {
+        st = hip_fail(e, "triangle upload");
+    } else if (G == 0) {
+        // no meshes: no BLAS
+    } else if (ctx->tune.bvh_builder == 3) {
+        // every mesh's tree on the GPU together
+        std::vector<int2> ranges(G);
+        std::vector<uint32_t> counts(G);
+        for (size_t g = 0; g < G; g++) ranges[g] = make_int2((int)tri_off[g], (int)mesh_tris[g]);
+        e = build_blas_forest(ctx->stream, W, ctx->tune.leaf_size, ranges, (uint32_t)ntri, d_soup, d_tris, d_shade_soup,
+                              d_shade, d_nodes, 0, ctx->bs, blas_root.data(), counts.data(), blas_box.data());
+        if (e != hipSuccess) st = hip_fail(e, "BLAS build");
+        for (uint32_t c : counts) node_base += c;
+        tri_base = (uint32_t)ntri;
+    } else {
+        for (size_t g = 0; g < G; g++) {
+            const uint32_t n = mesh_tris[g], off = tri_off[g];
+            uint32_t nn = 0;
+            e = build_blas(ctx->stream, W, ctx->tune.leaf_size, n, d_soup + 3 * size_t(off), d_tris,
+                           d_shade_soup + 4 * size_t(off), d_shade,
+                           ctx->tune.bvh_builder == 1 ? soup.data() + 3 * size_t(off) : nullptr, tri_base, d_nodes, node_base,
+                           ctx->bs, &nn, &blas_box[g * 6], ctx->tune.bvh_builder == 2, false);
+            if (e != hipSuccess) {
+                st = hip_fail(e, "BLAS build");
+                break;
+            }
+            blas_root[g] = node_base;
+            node_base += nn;
+            tri_base += n;
         }
-        blas_root[g] = node_base;
-        node_base += nn;
-        tri_base += n;
     }
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_soup);
@@ -1015,7 +1055,8 @@ hh_status hh_scene_commit(hh_scene s) {
         tlas_nodes = 1;
     } else {
         hipError_t e = build_tlas(ctx->stream, W, (uint32_t)s->instances.size(), ilo.data(), ihi.data(), d_nodes, node_base, ctx->bs,
-                                  &tlas_nodes, ctx->tune.bvh_builder == 1, ctx->tune.bvh_builder == 2);
+                                  &tlas_nodes, ctx->tune.bvh_builder == 1, ctx->tune.bvh_builder == 2,
+                                  ctx->tune.bvh_builder == 3);
         if (e != hipSuccess) return hip_fail(e, "TLAS build");
     }
     s->bvh_stats[2] = tlas_nodes;

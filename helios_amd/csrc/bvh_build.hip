@@ -366,6 +366,28 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Binned SAH arithmetic shared by the host builder and the GPU builder, so
+// both choose the same splits: 32 centroid bins per axis, bin of a centroid
+// c in [cl, ch] = trunc((c - cl) * (32 / (ch - cl))) capped at 31, half
+// surface area dx*dy + dy*dz + dz*dx, and the cost of a split = left count x
+// left area + right count x right area, evaluated in this order (the library
+// is built without floating-point contraction).
+// ---------------------------------------------------------------------------
+constexpr int kSahBins = 32;
+
+__host__ __device__ inline int sah_bin(float c, float cl, float scale) {
+    const float t = (c - cl) * scale;
+    return t < float(kSahBins - 1) ? int(t) : kSahBins - 1;   // NaN -> last bin
+}
+__host__ __device__ inline float sah_area(float lx, float ly, float lz, float hx, float hy, float hz) {
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    return dx * dy + dy * dz + dz * dx;
+}
+__host__ __device__ inline float sah_cost(int lc, float la, int rc, float ra) {
+    return float(lc) * la + float(rc) * ra;
+}
+
+// ---------------------------------------------------------------------------
 // PLOC: parallel locally-ordered clustering (Meister and Bittner, "Parallel
 // Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
 // IEEE TVCG 24(3), 2018). The clusters start as the primitives in Morton
@@ -524,6 +546,452 @@ __global__ void k_ploc_to_radix(uint32_t n, const int2* __restrict__ pchild, con
     if (x != root) parent[ref(x)] = root - pparent[x];
 }
 
+// ---------------------------------------------------------------------------
+// Binned SAH on the GPU (bvh_builder 3): the host builder's splits, computed
+// on the device, so the trees are the same. Ranges of more than kSahSmall
+// primitives are split level by level by block-wide passes over chunks of at
+// most kSahChunk positions: centroid bounds and bins (LDS, then global
+// atomics on order-preserving integer encodings of the floats, so min and
+// max are exact), the split choice (one wave per range), and a stable
+// partition (left counts per chunk, a scan over each range's chunks, block
+// prefix sums) into a second order buffer and back. Ranges of 2..kSahSmall
+// primitives are then built whole by one wave each, with the range's order
+// and its bins in LDS, children taken depth first (the larger one stacked,
+// so the stack holds at most log2(kSahSmall) entries).
+//
+// The split choice runs the host's two sweeps over the 32 bins of an axis as
+// inclusive scans across 32 lanes (sums of counts and min / max of bounds are
+// exact in any order), then takes the smallest cost, ties to the lowest
+// (axis, bin): the first strict minimum of the host's loop.
+// ---------------------------------------------------------------------------
+constexpr int kSahSmall = 1024;
+constexpr int kSahChunk = 4096;
+constexpr int kSahBinWords = 3 * kSahBins * 7;   // per axis and bin: count, lo xyz, hi xyz (encoded)
+constexpr int kSahStack = 16;                     // > log2(kSahSmall)
+
+struct SahSplit {
+    int axis, bin, m, mid;   // mid: split at the middle of the range (no partition)
+    float cl, scale;         // bin arithmetic of the split axis
+    int pad0, pad1;
+};
+
+__device__ inline float hmin(float a, float b) { return b < a ? b : a; }   // std::min
+__device__ inline float hmax(float a, float b) { return a < b ? b : a; }   // std::max
+
+__global__ void k_sah_centroids(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t n,
+                                float4* __restrict__ cen, uint32_t* __restrict__ order) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 l = lo[i], h = hi[i];
+    cen[i] = make_float4(0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z), 0.f);
+    order[i] = i;
+}
+
+__device__ inline void sah_bins_init(uint32_t* w, int i) {
+    const int f = i % 7;
+    w[i] = f == 0 ? 0u : (f < 4 ? f2o(INFINITY) : f2o(-INFINITY));
+}
+
+// Centroid bounds (6 encoded words) and bins of every range of a level.
+__global__ void k_sah_level_init(uint32_t nn, uint32_t* __restrict__ cb, uint32_t* __restrict__ bins) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn * 6) cb[i] = (i % 6) < 3 ? f2o(INFINITY) : f2o(-INFINITY);
+    if (i < nn * kSahBinWords) sah_bins_init(bins, (int)i);
+}
+
+__global__ void __launch_bounds__(256) k_sah_bounds(const int4* __restrict__ chunks, const uint32_t* __restrict__ order,
+                                                    const float4* __restrict__ cen, uint32_t* __restrict__ cb) {
+    const int4 ch = chunks[blockIdx.x];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = ch.y + (int)threadIdx.x; i < ch.z; i += 256) {
+        float4 c = cen[order[i]];
+        mn[0] = fminf(mn[0], c.x); mn[1] = fminf(mn[1], c.y); mn[2] = fminf(mn[2], c.z);
+        mx[0] = fmaxf(mx[0], c.x); mx[1] = fmaxf(mx[1], c.y); mx[2] = fmaxf(mx[2], c.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], off));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], off));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&cb[6 * ch.x + k], f2o(mn[k]));
+            atomicMax(&cb[6 * ch.x + 3 + k], f2o(mx[k]));
+        }
+}
+
+// Adds one primitive to the encoded bins `w` of its range.
+__device__ inline void sah_bin_prim(uint32_t* w, const float cl[3], const float scale[3], const bool live[3], float4 c,
+                                    float4 l, float4 h) {
+    const float cc[3] = {c.x, c.y, c.z};
+    for (int k = 0; k < 3; k++) {
+        if (!live[k]) continue;
+        uint32_t* b = w + (k * kSahBins + sah_bin(cc[k], cl[k], scale[k])) * 7;
+        atomicAdd(&b[0], 1u);
+        atomicMin(&b[1], f2o(l.x)); atomicMin(&b[2], f2o(l.y)); atomicMin(&b[3], f2o(l.z));
+        atomicMax(&b[4], f2o(h.x)); atomicMax(&b[5], f2o(h.y)); atomicMax(&b[6], f2o(h.z));
+    }
+}
+
+__device__ inline void sah_axes(const float cl[3], const float ch[3], float scale[3], bool live[3]) {
+    for (int k = 0; k < 3; k++) {
+        const float ext = ch[k] - cl[k];
+        live[k] = ext > 0.0f;
+        scale[k] = live[k] ? float(kSahBins) / ext : 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sah_bin(const int4* __restrict__ chunks, const uint32_t* __restrict__ order,
+                                                 const float4* __restrict__ cen, const float4* __restrict__ lo,
+                                                 const float4* __restrict__ hi, const uint32_t* __restrict__ cb,
+                                                 uint32_t* __restrict__ bins) {
+    __shared__ uint32_t w[kSahBinWords];
+    const int4 ch = chunks[blockIdx.x];
+    for (int i = threadIdx.x; i < kSahBinWords; i += 256) sah_bins_init(w, i);
+    float cl[3], cu[3], scale[3];
+    bool live[3];
+    for (int k = 0; k < 3; k++) {
+        cl[k] = o2f(cb[6 * ch.x + k]);
+        cu[k] = o2f(cb[6 * ch.x + 3 + k]);
+    }
+    sah_axes(cl, cu, scale, live);
+    __syncthreads();
+    for (int i = ch.y + (int)threadIdx.x; i < ch.z; i += 256) {
+        uint32_t p = order[i];
+        sah_bin_prim(w, cl, scale, live, cen[p], lo[p], hi[p]);
+    }
+    __syncthreads();
+    uint32_t* g = bins + (size_t)ch.x * kSahBinWords;
+    for (int i = threadIdx.x; i < kSahBinWords; i += 256) {
+        const uint32_t v = w[i];
+        const int f = i % 7;
+        if (f == 0) {
+            if (v) atomicAdd(&g[i], v);
+        } else if (f < 4) {
+            if (v != f2o(INFINITY)) atomicMin(&g[i], v);
+        } else if (v != f2o(-INFINITY)) {
+            atomicMax(&g[i], v);
+        }
+    }
+}
+
+// The split of one range from its encoded bins `w` (LDS) and its centroid
+// bounds, by the 64 lanes of one wave; every lane returns the same choice.
+// axis < 0: no split with a finite cost (coincident centroids).
+struct SahChoice {
+    int axis, bin, nleft;
+};
+__device__ SahChoice sah_choose_wave(const uint32_t* w, const float cl[3], const float cu[3]) {
+    const int lane = (int)__lane_id();
+    const int q = lane & (kSahBins - 1);
+    const bool act = lane < kSahBins;
+    float best = INFINITY;
+    int bidx = -1;
+    for (int k = 0; k < 3; k++) {
+        const float ext = cu[k] - cl[k];
+        if (!(ext > 0.0f)) continue;
+        const uint32_t* b = w + (k * kSahBins + q) * 7;
+        const int cnt = act ? (int)b[0] : 0;
+        const float lx = act ? o2f(b[1]) : INFINITY, ly = act ? o2f(b[2]) : INFINITY, lz = act ? o2f(b[3]) : INFINITY;
+        const float hx = act ? o2f(b[4]) : -INFINITY, hy = act ? o2f(b[5]) : -INFINITY, hz = act ? o2f(b[6]) : -INFINITY;
+        // right side: bins q..31 (suffix), left side: bins 0..q (prefix)
+        int rc = cnt, lc = cnt;
+        float rlx = lx, rly = ly, rlz = lz, rhx = hx, rhy = hy, rhz = hz;
+        float llx = lx, lly = ly, llz = lz, lhx = hx, lhy = hy, lhz = hz;
+        for (int off = 1; off < kSahBins; off <<= 1) {
+            const int orc = __shfl_down(rc, off);
+            const float olx = __shfl_down(rlx, off), oly = __shfl_down(rly, off), olz = __shfl_down(rlz, off);
+            const float ohx = __shfl_down(rhx, off), ohy = __shfl_down(rhy, off), ohz = __shfl_down(rhz, off);
+            if (q + off < kSahBins) {
+                rc += orc;
+                rlx = hmin(rlx, olx); rly = hmin(rly, oly); rlz = hmin(rlz, olz);
+                rhx = hmax(rhx, ohx); rhy = hmax(rhy, ohy); rhz = hmax(rhz, ohz);
+            }
+            const int plc = __shfl_up(lc, off);
+            const float plx = __shfl_up(llx, off), ply = __shfl_up(lly, off), plz = __shfl_up(llz, off);
+            const float phx = __shfl_up(lhx, off), phy = __shfl_up(lhy, off), phz = __shfl_up(lhz, off);
+            if (q >= off) {
+                lc += plc;
+                llx = hmin(llx, plx); lly = hmin(lly, ply); llz = hmin(llz, plz);
+                lhx = hmax(lhx, phx); lhy = hmax(lhy, phy); lhz = hmax(lhz, phz);
+            }
+        }
+        const float rarea = rc ? sah_area(rlx, rly, rlz, rhx, rhy, rhz) : 0.0f;
+        const int rc_next = __shfl_down(rc, 1);
+        const float ra_next = __shfl_down(rarea, 1);
+        float cost = INFINITY;
+        if (act && q < kSahBins - 1 && lc != 0 && rc_next != 0)
+            cost = sah_cost(lc, sah_area(llx, lly, llz, lhx, lhy, lhz), rc_next, ra_next);
+        if (!(cost < INFINITY)) cost = INFINITY;   // NaN or infinite: never chosen
+        int bq = cost < INFINITY ? q : 1 << 30;
+        for (int off = 32; off > 0; off >>= 1) {
+            const float oc = __shfl_xor(cost, off);
+            const int oq = __shfl_xor(bq, off);
+            if (oc < cost || (oc == cost && oq < bq)) {
+                cost = oc;
+                bq = oq;
+            }
+        }
+        if (bq < (1 << 30) && cost < best) {   // strict: an earlier axis keeps a tie
+            best = cost;
+            bidx = k * kSahBins + bq;
+        }
+    }
+    SahChoice r{-1, -1, 0};
+    if (bidx >= 0) {
+        r.axis = bidx / kSahBins;
+        r.bin = bidx % kSahBins;
+        int v = (act && q <= r.bin) ? (int)w[(r.axis * kSahBins + q) * 7] : 0;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        r.nleft = v;
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(64) k_sah_choose(const int4* __restrict__ nodes, const uint32_t* __restrict__ cb,
+                                                   const uint32_t* __restrict__ bins, SahSplit* __restrict__ splits) {
+    __shared__ uint32_t w[kSahBinWords];
+    const int j = blockIdx.x;
+    for (int i = threadIdx.x; i < kSahBinWords; i += 64) w[i] = bins[(size_t)j * kSahBinWords + i];
+    __syncthreads();
+    float cl[3], cu[3];
+    for (int k = 0; k < 3; k++) {
+        cl[k] = o2f(cb[6 * j + k]);
+        cu[k] = o2f(cb[6 * j + 3 + k]);
+    }
+    const SahChoice c = sah_choose_wave(w, cl, cu);
+    if (threadIdx.x == 0) {
+        const int4 nd = nodes[j];
+        SahSplit s{};
+        s.axis = c.axis;
+        s.bin = c.bin;
+        s.m = nd.x + c.nleft;
+        s.mid = 0;
+        if (c.axis < 0 || !(s.m > nd.x && s.m < nd.y)) {
+            s.mid = 1;
+            s.m = nd.x + (nd.y - nd.x) / 2;
+        } else {
+            s.cl = cl[c.axis];
+            s.scale = float(kSahBins) / (cu[c.axis] - cl[c.axis]);
+        }
+        splits[j] = s;
+    }
+}
+
+__device__ inline bool sah_goes_left(const SahSplit& s, int i, const float4* cen, uint32_t p) {
+    if (s.mid) return i < s.m;
+    const float4 c = cen[p];
+    const float v = s.axis == 0 ? c.x : (s.axis == 1 ? c.y : c.z);
+    return sah_bin(v, s.cl, s.scale) <= s.bin;
+}
+
+__global__ void __launch_bounds__(256) k_sah_count(const int4* __restrict__ chunks, const SahSplit* __restrict__ splits,
+                                                   const uint32_t* __restrict__ order, const float4* __restrict__ cen,
+                                                   uint32_t* __restrict__ chunk_left) {
+    __shared__ uint32_t part[4];
+    const int4 ch = chunks[blockIdx.x];
+    const SahSplit s = splits[ch.x];
+    uint32_t cnt = 0;
+    for (int i = ch.y + (int)threadIdx.x; i < ch.z; i += 256) cnt += sah_goes_left(s, i, cen, order[i]) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_left[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Per range: the left count before each of its chunks.
+__global__ void k_sah_chunkscan(uint32_t nn, const int2* __restrict__ node_chunks, const uint32_t* __restrict__ chunk_left,
+                                uint32_t* __restrict__ chunk_base) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nn) return;
+    const int2 nc = node_chunks[j];
+    uint32_t run = 0;
+    for (int c = nc.x; c < nc.x + nc.y; c++) {
+        chunk_base[c] = run;
+        run += chunk_left[c];
+    }
+}
+
+// Stable partition of each chunk's positions into `out` (left side from the
+// range's start, right side from its split position).
+__global__ void __launch_bounds__(256) k_sah_scatter(const int4* __restrict__ chunks, const int4* __restrict__ nodes,
+                                                     const SahSplit* __restrict__ splits, const uint32_t* __restrict__ order,
+                                                     const float4* __restrict__ cen, const uint32_t* __restrict__ chunk_base,
+                                                     uint32_t* __restrict__ out) {
+    __shared__ uint32_t wsum[4];
+    const int4 ch = chunks[blockIdx.x];
+    const int b = nodes[ch.x].x;
+    const SahSplit s = splits[ch.x];
+    const int lane = (int)__lane_id(), wv = (int)threadIdx.x >> 6;
+    uint32_t run = chunk_base[blockIdx.x];
+    for (int base = ch.y; base < ch.z; base += 256) {
+        const int i = base + (int)threadIdx.x;
+        const bool in = i < ch.z;
+        const uint32_t p = in ? order[i] : 0u;
+        const bool left = in && sah_goes_left(s, i, cen, p);
+        const uint64_t bal = __ballot(left);
+        const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = run;
+        for (int k = 0; k < wv; k++) before += wsum[k];
+        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+        if (in) {
+            const uint32_t lr = before + below;   // left flags in [b, i)
+            const int pos = left ? b + (int)lr : s.m + (i - b - (int)lr);
+            out[pos] = p;
+        }
+        run += total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sah_copyback(const int4* __restrict__ chunks, const uint32_t* __restrict__ src,
+                                                      uint32_t* __restrict__ dst) {
+    const int4 ch = chunks[blockIdx.x];
+    for (int i = ch.y + (int)threadIdx.x; i < ch.z; i += 256) dst[i] = src[i];
+}
+
+// Children of the binary node `id` over [b, e) split at m (the host
+// builder's numbering: left child id + 1, right child id + (m - b), a
+// one-primitive side is the leaf ref n - 1 + position).
+__device__ inline void sah_emit_node(int b, int e, int id, int m, int n, int2* childs, int2* range, uint32_t* parent) {
+    const int lid = id + 1, rid = id + (m - b);
+    const int l = m - b == 1 ? n - 1 + b : lid;
+    const int r = e - m == 1 ? n - 1 + m : rid;
+    childs[id] = make_int2(l, r);
+    range[id] = make_int2(b, e - 1);
+    parent[l] = (uint32_t)id;
+    parent[r] = (uint32_t)id;
+}
+
+// Ranges are (first position, end position, binary node id, mesh); a
+// mesh's primitives hold positions P..P+n-1 (meshes[mesh] = (P, n)), and its
+// tree is written in its own numbering to childs + P, range + P and
+// parent + 2P.
+__device__ inline void sah_emit_mesh(int4 nd, int m, const int2* meshes, int2* childs, int2* range, uint32_t* parent) {
+    const int2 mp = meshes[nd.w];
+    sah_emit_node(nd.x - mp.x, nd.y - mp.x, nd.z, m - mp.x, mp.y, childs + mp.x, range + mp.x, parent + 2 * (size_t)mp.x);
+}
+
+__global__ void k_sah_emit(uint32_t nn, const int4* __restrict__ nodes, const SahSplit* __restrict__ splits,
+                           const int2* __restrict__ meshes, int2* __restrict__ childs, int2* __restrict__ range,
+                           uint32_t* __restrict__ parent, int4* __restrict__ next, uint32_t* __restrict__ next_count,
+                           int4* __restrict__ small, uint32_t* __restrict__ small_count) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nn) return;
+    const int4 nd = nodes[j];
+    const int m = splits[j].m;
+    sah_emit_mesh(nd, m, meshes, childs, range, parent);
+    const int4 kids[2] = {make_int4(nd.x, m, nd.z + 1, nd.w), make_int4(m, nd.y, nd.z + (m - nd.x), nd.w)};
+    for (int c = 0; c < 2; c++) {
+        const int sz = kids[c].y - kids[c].x;
+        if (sz > kSahSmall) next[atomicAdd(next_count, 1u)] = kids[c];
+        else if (sz >= 2) small[atomicAdd(small_count, 1u)] = kids[c];
+    }
+}
+
+// One wave builds the whole subtree over one range of 2..kSahSmall
+// primitives. The range's centroids and boxes are staged in LDS once (slot j
+// = the j-th primitive of the range), and the order is kept as slots.
+__global__ void __launch_bounds__(64) k_sah_small(const int4* __restrict__ ranges, const int2* __restrict__ meshes,
+                                                  const float4* __restrict__ cen,
+                                                  const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                  uint32_t* __restrict__ order, int2* __restrict__ childs,
+                                                  int2* __restrict__ range, uint32_t* __restrict__ parent) {
+    __shared__ uint16_t s_ord[kSahSmall], s_tmp[kSahSmall];
+    __shared__ uint32_t s_gid[kSahSmall];
+    __shared__ float s_c[3][kSahSmall], s_lo[3][kSahSmall], s_hi[3][kSahSmall];
+    __shared__ uint32_t w[kSahBinWords];
+    __shared__ int4 stack[kSahStack];
+    const int lane = (int)threadIdx.x;
+    const int4 root = ranges[blockIdx.x];
+    const int b0 = root.x, S = root.y - root.x;
+    for (int j = lane; j < S; j += 64) {
+        const uint32_t p = order[b0 + j];
+        const float4 c = cen[p], l = lo[p], h = hi[p];
+        s_gid[j] = p;
+        s_ord[j] = (uint16_t)j;
+        s_c[0][j] = c.x; s_c[1][j] = c.y; s_c[2][j] = c.z;
+        s_lo[0][j] = l.x; s_lo[1][j] = l.y; s_lo[2][j] = l.z;
+        s_hi[0][j] = h.x; s_hi[1][j] = h.y; s_hi[2][j] = h.z;
+    }
+    __syncthreads();
+    int cb = root.x, ce = root.y, cid = root.z, sp = 0;
+    for (;;) {
+        // cb, ce, cid, sp are uniform across the wave
+        if (ce - cb >= 2) {
+            float cl[3] = {INFINITY, INFINITY, INFINITY}, cu[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int j = cb - b0 + lane; j < ce - b0; j += 64) {
+                const int t = s_ord[j];
+                for (int k = 0; k < 3; k++) {
+                    cl[k] = fminf(cl[k], s_c[k][t]);
+                    cu[k] = fmaxf(cu[k], s_c[k][t]);
+                }
+            }
+            for (int k = 0; k < 3; k++)
+                for (int off = 32; off > 0; off >>= 1) {
+                    cl[k] = fminf(cl[k], __shfl_xor(cl[k], off));
+                    cu[k] = fmaxf(cu[k], __shfl_xor(cu[k], off));
+                }
+            float scale[3];
+            bool live[3];
+            sah_axes(cl, cu, scale, live);
+            for (int i = lane; i < kSahBinWords; i += 64) sah_bins_init(w, i);
+            __syncthreads();
+            for (int j = cb - b0 + lane; j < ce - b0; j += 64) {
+                const int t = s_ord[j];
+                sah_bin_prim(w, cl, scale, live, make_float4(s_c[0][t], s_c[1][t], s_c[2][t], 0.f),
+                             make_float4(s_lo[0][t], s_lo[1][t], s_lo[2][t], 0.f),
+                             make_float4(s_hi[0][t], s_hi[1][t], s_hi[2][t], 0.f));
+            }
+            __syncthreads();
+            const SahChoice c = sah_choose_wave(w, cl, cu);
+            int m = cb + c.nleft;
+            const bool mid = c.axis < 0 || !(m > cb && m < ce);
+            if (mid) {
+                m = cb + (ce - cb) / 2;
+            } else {
+                const float scl = float(kSahBins) / (cu[c.axis] - cl[c.axis]);
+                uint32_t run = 0;
+                for (int base = cb; base < ce; base += 64) {
+                    const int i = base + lane;
+                    const bool in = i < ce;
+                    const int t = in ? s_ord[i - b0] : 0;
+                    const bool left = in && sah_bin(s_c[c.axis][t], cl[c.axis], scl) <= c.bin;
+                    const uint64_t bal = __ballot(left);
+                    const uint32_t lr = run + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    if (in) s_tmp[(left ? cb + (int)lr : m + (i - cb - (int)lr)) - b0] = (uint16_t)t;
+                    run += (uint32_t)__popcll(bal);
+                }
+                __syncthreads();
+                for (int j = cb - b0 + lane; j < ce - b0; j += 64) s_ord[j] = s_tmp[j];
+                __syncthreads();
+            }
+            if (lane == 0) sah_emit_mesh(make_int4(cb, ce, cid, root.w), m, meshes, childs, range, parent);
+            const int ls = m - cb, rs = ce - m;
+            const int4 L = make_int4(cb, m, cid + 1, 0), R = make_int4(m, ce, cid + ls, 0);
+            if (ls >= 2 && rs >= 2) {
+                const int4 big = ls >= rs ? L : R, sml = ls >= rs ? R : L;
+                if (lane == 0) stack[sp] = big;
+                sp++;
+                cb = sml.x; ce = sml.y; cid = sml.z;
+                __syncthreads();
+                continue;
+            }
+            if (ls >= 2) { cb = L.x; ce = L.y; cid = L.z; continue; }
+            if (rs >= 2) { cb = R.x; ce = R.y; cid = R.z; continue; }
+        }
+        if (sp == 0) break;
+        sp--;
+        const int4 t = stack[sp];
+        cb = t.x; ce = t.y; cid = t.z;
+    }
+    __syncthreads();
+    for (int j = lane; j < S; j += 64) order[b0 + j] = s_gid[s_ord[j]];
+}
+
 inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 }  // namespace
@@ -540,11 +1008,11 @@ inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 // 0..n-2, leaf j = binary ref n-1+j over the primitive order `order`,
 // contiguous ranges, parents), so the GPU refit and the wide collapse are
 // shared. Every split is chosen among 32 centroid bins on each axis by the
-// surface-area cost (leaf of m primitives costs m, a split costs its
-// children's area-weighted counts); ranges whose centroids coincide, or
-// whose best split would leave a side empty, are split at the median. The
-// subtree over m primitives owns the m - 1 internal ids after its root, so
-// subtrees are built in parallel without shared state.
+// surface-area cost (a split costs its children's area-weighted counts),
+// and the range is partitioned stably, so every range stays in ascending
+// primitive order; ranges whose centroids coincide are split in the middle.
+// The subtree over m primitives owns the m - 1 internal ids after its root,
+// so subtrees are built in parallel without shared state.
 // ---------------------------------------------------------------------------
 namespace {
 struct SahPrim {
@@ -558,10 +1026,7 @@ struct SahTree {
     std::vector<uint32_t> parent;
     int n = 0;
 
-    static float area(const float lo[3], const float hi[3]) {
-        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-        return dx * dy + dy * dz + dz * dx;
-    }
+    static float area(const float lo[3], const float hi[3]) { return sah_area(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]); }
 
     // split [b, e) and return the first index of the right side
     int split(int b, int e) {
@@ -572,7 +1037,7 @@ struct SahTree {
                 cl[k] = std::min(cl[k], P[order[i]].c[k]);
                 ch[k] = std::max(ch[k], P[order[i]].c[k]);
             }
-        constexpr int NB = 32;
+        constexpr int NB = kSahBins;
         float best = INFINITY;
         int best_axis = -1, best_bin = -1;
         for (int k = 0; k < 3; k++) {
@@ -585,7 +1050,7 @@ struct SahTree {
                 for (int a = 0; a < 3; a++) { blo[q][a] = INFINITY; bhi[q][a] = -INFINITY; }
             for (int i = b; i < e; i++) {
                 const SahPrim& p = P[order[i]];
-                const int q = std::min(NB - 1, int((p.c[k] - cl[k]) * scale));
+                const int q = sah_bin(p.c[k], cl[k], scale);
                 cnt[q]++;
                 for (int a = 0; a < 3; a++) {
                     blo[q][a] = std::min(blo[q][a], p.lo[a]);
@@ -608,26 +1073,22 @@ struct SahTree {
                 lc += cnt[q];
                 for (int a = 0; a < 3; a++) { ll[a] = std::min(ll[a], blo[q][a]); lh[a] = std::max(lh[a], bhi[q][a]); }
                 if (lc == 0 || rcnt[q + 1] == 0) continue;
-                const float cost = lc * area(ll, lh) + rcnt[q + 1] * rarea[q + 1];
+                const float cost = sah_cost(lc, area(ll, lh), rcnt[q + 1], rarea[q + 1]);
                 if (cost < best) { best = cost; best_axis = k; best_bin = q; }
             }
         }
         if (best_axis >= 0) {
             const float ext = ch[best_axis] - cl[best_axis], scale = NB / ext;
-            auto mid = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t i) {
-                return std::min(NB - 1, int((P[i].c[best_axis] - cl[best_axis]) * scale)) <= best_bin;
+            // stable: every range stays in ascending primitive order
+            auto mid = std::stable_partition(order.begin() + b, order.begin() + e, [&](uint32_t i) {
+                return sah_bin(P[i].c[best_axis], cl[best_axis], scale) <= best_bin;
             });
             const int m = int(mid - order.begin());
             if (m > b && m < e) return m;
         }
-        // coincident centroids or a one-sided split: median on the widest axis
-        int ax = 0;
-        for (int k = 1; k < 3; k++)
-            if (ch[k] - cl[k] > ch[ax] - cl[ax]) ax = k;
-        const int m = b + (e - b) / 2;
-        std::nth_element(order.begin() + b, order.begin() + m, order.begin() + e,
-                         [&](uint32_t x, uint32_t y) { return P[x].c[ax] < P[y].c[ax]; });
-        return m;
+        // coincident centroids (or no finite cost): the middle of the range,
+        // which is in ascending primitive order
+        return b + (e - b) / 2;
     }
 
     int ref_of(int b, int e, int id) const { return e - b == 1 ? n - 1 + b : id; }
@@ -675,14 +1136,67 @@ static void sah_tree(uint32_t n, const float* lo, const float* hi, SahTree& t, s
     if (n > 1) t.build(0, int(n), 0, 0);
 }
 
+static hipError_t sah_forest_gpu(hipStream_t st, const std::vector<int2>& meshes, uint32_t total, const float4* d_lo,
+                                 const float4* d_hi, uint32_t* d_order, int2* childs, int2* range, uint32_t* parent,
+                                 BuildScratch& sc);
+
+// Bounds (refit) and the W-wide collapse of one binary tree in the radix
+// tree's form over n primitives: `order` maps leaf positions to indices into
+// d_lo / d_hi; childs / range / flags hold n - 1 entries, parent / blo / bhi
+// 2n - 1. Nodes go to d_nodes (global index node_base + i), leaves reference
+// leaf_base + position (or the primitive when leaf_is_prim).
+template <int W>
+static hipError_t finish_wide(hipStream_t st, uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_order,
+                              const int2* childs, const int2* range, const uint32_t* parent, uint32_t* flags, float4* blo,
+                              float4* bhi, BuildScratch& sc, int leaf_size, int leaf_is_prim, WideNode<W>* d_nodes,
+                              uint32_t node_base, uint32_t leaf_base, uint32_t* num_nodes, float root_box[6]) {
+    const uint32_t B = 256;
+    k_refit<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, d_order, (int)n, childs, parent, flags, blo, bhi);
+    int2 job0 = make_int2(0, 0);   // binary ref of the root (leaf 0 when n == 1: ref n-1+0 = 0)
+    uint32_t one = 1;
+    HH_TRY(hipMemcpyAsync(sc.jobs_a, &job0, sizeof(int2), hipMemcpyHostToDevice, st));
+    HH_TRY(hipMemcpyAsync(sc.counters, &one, 4, hipMemcpyHostToDevice, st));       // jobs in A
+    HH_TRY(hipMemcpyAsync(sc.counters + 2, &one, 4, hipMemcpyHostToDevice, st));   // node counter
+    uint32_t njobs = 1;
+    int2* qa = sc.jobs_a;
+    int2* qb = sc.jobs_b;
+    uint32_t* ca = sc.counters;
+    uint32_t* cb = sc.counters + 1;
+    while (njobs > 0) {
+        k_fill_u32<<<1, 1, 0, st>>>(cb, 0u, 1);
+        k_collapse<W><<<blocks(njobs, 64), 64, 0, st>>>(qa, ca, qb, cb, sc.counters + 2, (int)n, leaf_size, childs, range, blo,
+                                                         bhi, d_order, d_nodes, node_base, leaf_base, leaf_is_prim);
+        HH_TRY(hipMemcpyAsync(&njobs, cb, 4, hipMemcpyDeviceToHost, st));
+        HH_TRY(hipStreamSynchronize(st));
+        std::swap(qa, qb);
+        std::swap(ca, cb);
+    }
+    HH_TRY(hipMemcpyAsync(num_nodes, sc.counters + 2, 4, hipMemcpyDeviceToHost, st));
+    float4 rb[2];
+    HH_TRY(hipMemcpyAsync(&rb[0], &blo[0], 16, hipMemcpyDeviceToHost, st));
+    HH_TRY(hipMemcpyAsync(&rb[1], &bhi[0], 16, hipMemcpyDeviceToHost, st));
+    HH_TRY(hipStreamSynchronize(st));
+    root_box[0] = rb[0].x; root_box[1] = rb[0].y; root_box[2] = rb[0].z;
+    root_box[3] = rb[1].x; root_box[4] = rb[1].y; root_box[5] = rb[1].z;
+    return hipSuccess;
+}
+
 template <int W>
 static hipError_t build_wide(hipStream_t st, uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_cbounds,
                              WideNode<W>* d_nodes, uint32_t node_base, uint32_t leaf_base, int leaf_size, int leaf_is_prim,
                              uint32_t* d_order, BuildScratch& sc, uint32_t* num_nodes, float root_box[6],
-                             const float* h_lo = nullptr, const float* h_hi = nullptr, bool ploc = false) {
+                             const float* h_lo = nullptr, const float* h_hi = nullptr, bool ploc = false,
+                             bool sah_gpu = false) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
-    if (ploc && n > 1) {
+    if (sah_gpu && n > 1) {
+        HH_TRY(sc.ensure_sah(n));
+        const std::vector<int2> one_mesh{make_int2(0, (int)n)};
+        HH_TRY(sah_forest_gpu(st, one_mesh, n, d_lo, d_hi, d_order, sc.childs, sc.range, sc.parent, sc));
+        k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
+    } else if (sah_gpu) {
+        k_fill_u32<<<1, 1, 0, st>>>(d_order, 0u, 1);
+    } else if (ploc && n > 1) {
         HH_TRY(sc.ensure_ploc(n));
         k_morton<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, n, d_cbounds, sc.keys_in, sc.vals_in);
         size_t tb = sc.sort_bytes;
@@ -739,35 +1253,129 @@ static hipError_t build_wide(hipStream_t st, uint32_t n, const float4* d_lo, con
             k_fill_u32<<<blocks(n, B), B, 0, st>>>(sc.flags, 0u, n);
         }
     }
-    k_refit<<<blocks(n, B), B, 0, st>>>(d_lo, d_hi, d_order, (int)n, sc.childs, sc.parent, sc.flags, sc.blo, sc.bhi);
-    int root = n > 1 ? 0 : 0;  // binary ref of the root (leaf 0 when n == 1: ref n-1+0 = 0)
-    int2 job0 = make_int2(root, 0);
-    uint32_t one = 1;
-    HH_TRY(hipMemcpyAsync(sc.jobs_a, &job0, sizeof(int2), hipMemcpyHostToDevice, st));
-    HH_TRY(hipMemcpyAsync(sc.counters, &one, 4, hipMemcpyHostToDevice, st));       // jobs in A
-    HH_TRY(hipMemcpyAsync(sc.counters + 2, &one, 4, hipMemcpyHostToDevice, st));   // node counter
-    uint32_t njobs = 1;
-    int2* qa = sc.jobs_a;
-    int2* qb = sc.jobs_b;
-    uint32_t* ca = sc.counters;
-    uint32_t* cb = sc.counters + 1;
-    while (njobs > 0) {
-        k_fill_u32<<<1, 1, 0, st>>>(cb, 0u, 1);
-        k_collapse<W><<<blocks(njobs, 64), 64, 0, st>>>(qa, ca, qb, cb, sc.counters + 2, (int)n, leaf_size, sc.childs, sc.range,
-                                                         sc.blo, sc.bhi, d_order, d_nodes, node_base, leaf_base, leaf_is_prim);
-        HH_TRY(hipMemcpyAsync(&njobs, cb, 4, hipMemcpyDeviceToHost, st));
-        HH_TRY(hipStreamSynchronize(st));
-        std::swap(qa, qb);
-        std::swap(ca, cb);
-    }
-    HH_TRY(hipMemcpyAsync(num_nodes, sc.counters + 2, 4, hipMemcpyDeviceToHost, st));
-    float4 rb[2];
-    HH_TRY(hipMemcpyAsync(&rb[0], &sc.blo[0], 16, hipMemcpyDeviceToHost, st));
-    HH_TRY(hipMemcpyAsync(&rb[1], &sc.bhi[0], 16, hipMemcpyDeviceToHost, st));
-    HH_TRY(hipStreamSynchronize(st));
-    root_box[0] = rb[0].x; root_box[1] = rb[0].y; root_box[2] = rb[0].z;
-    root_box[3] = rb[1].x; root_box[4] = rb[1].y; root_box[5] = rb[1].z;
+    return finish_wide<W>(st, n, d_lo, d_hi, d_order, sc.childs, sc.range, sc.parent, sc.flags, sc.blo, sc.bhi, sc, leaf_size,
+                          leaf_is_prim, d_nodes, node_base, leaf_base, num_nodes, root_box);
+}
+
+hipError_t BuildScratch::ensure_sah(uint32_t n) {
+    if (n <= s_cap) return hipSuccess;
+    release_sah();
+    s_cap = std::max<uint32_t>(n, 1024);
+    const size_t N = s_cap;
+    s_nmax = s_cap / kSahSmall + 2;                      // disjoint ranges of > kSahSmall primitives
+    s_cmax = s_cap / kSahChunk + 2 * s_nmax + 2;         // chunks of <= kSahChunk positions per range
+    HH_TRY(hipMalloc(&s_cen, N * sizeof(float4)));
+    HH_TRY(hipMalloc(&s_ord2, N * 4));
+    HH_TRY(hipMalloc(&s_cb, size_t(s_nmax) * 6 * 4));
+    HH_TRY(hipMalloc(&s_bins, size_t(s_nmax) * kSahBinWords * 4));
+    HH_TRY(hipMalloc(&s_splits, size_t(s_nmax) * sizeof(SahSplit)));
+    HH_TRY(hipMalloc(&s_nodes, size_t(s_nmax) * sizeof(int4)));
+    HH_TRY(hipMalloc(&s_next, size_t(s_nmax) * sizeof(int4)));
+    HH_TRY(hipMalloc(&s_small, (N / 2 + 1) * sizeof(int4)));   // disjoint ranges of >= 2 primitives
+    HH_TRY(hipMalloc(&s_chunks, size_t(s_cmax) * sizeof(int4)));
+    HH_TRY(hipMalloc(&s_node_chunks, size_t(s_nmax) * sizeof(int2)));
+    HH_TRY(hipMalloc(&s_chunk_left, size_t(s_cmax) * 4));
+    HH_TRY(hipMalloc(&s_chunk_base, size_t(s_cmax) * 4));
+    HH_TRY(hipMalloc(&s_ctr, 16));
+    HH_TRY(hipHostMalloc(&s_tail, 16));
     return hipSuccess;
+}
+
+hipError_t BuildScratch::ensure_sah_meshes(uint32_t nm) {
+    if (nm <= s_mesh_cap) return hipSuccess;
+    if (s_meshes) (void)hipFree(s_meshes);
+    s_meshes = nullptr;
+    s_mesh_cap = std::max<uint32_t>(nm, 64);
+    HH_TRY(hipMalloc(&s_meshes, size_t(s_mesh_cap) * sizeof(int2)));
+    return hipSuccess;
+}
+
+void BuildScratch::release_sah() {
+    void* ps[] = {s_cen, s_ord2, s_cb, s_bins, s_splits, s_nodes, s_next, s_small, s_chunks, s_node_chunks, s_chunk_left,
+                  s_chunk_base, s_ctr};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (s_tail) (void)hipHostFree(s_tail);
+    if (s_meshes) (void)hipFree(s_meshes);
+    s_meshes = nullptr;
+    s_mesh_cap = 0;
+    s_cen = nullptr;
+    s_ord2 = s_cb = s_bins = s_chunk_left = s_chunk_base = s_ctr = s_tail = nullptr;
+    s_nodes = s_next = s_small = s_chunks = nullptr;
+    s_node_chunks = nullptr;
+    s_splits = nullptr;
+    s_cap = s_nmax = s_cmax = 0;
+}
+
+// Binned-SAH binary trees of several meshes at once on the GPU, each in the
+// radix tree's form: mesh g = (P, n) owns primitives / positions P..P+n-1
+// of d_lo / d_hi / d_order (d_order receives primitive indices) and writes
+// its tree to childs + P, range + P, parent + 2P in its own numbering.
+static hipError_t sah_forest_gpu(hipStream_t st, const std::vector<int2>& meshes, uint32_t total, const float4* d_lo,
+                                 const float4* d_hi, uint32_t* d_order, int2* childs, int2* range, uint32_t* parent,
+                                 BuildScratch& sc) {
+    HH_TRY(sc.ensure_sah(total));
+    HH_TRY(sc.ensure_sah_meshes((uint32_t)meshes.size()));
+    const uint32_t B = 256;
+    SahSplit* splits = (SahSplit*)sc.s_splits;
+    k_sah_centroids<<<blocks(total, B), B, 0, st>>>(d_lo, d_hi, total, sc.s_cen, d_order);
+    std::vector<int4> big, small0, chunks;
+    std::vector<int2> node_chunks;
+    for (size_t g = 0; g < meshes.size(); g++) {
+        const int4 root = make_int4(meshes[g].x, meshes[g].x + meshes[g].y, 0, (int)g);
+        if (meshes[g].y > kSahSmall) big.push_back(root);
+        else if (meshes[g].y >= 2) small0.push_back(root);
+    }
+    uint32_t ctr0[4] = {0u, (uint32_t)small0.size(), 0u, 0u};   // [0] next big ranges, [1] small ranges
+    HH_TRY(hipMemcpyAsync(sc.s_meshes, meshes.data(), meshes.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    HH_TRY(hipMemcpyAsync(sc.s_ctr, ctr0, sizeof(ctr0), hipMemcpyHostToDevice, st));
+    if (!small0.empty())
+        HH_TRY(hipMemcpyAsync(sc.s_small, small0.data(), small0.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    HH_TRY(hipStreamSynchronize(st));   // the host arrays above are reused
+    while (!big.empty()) {
+        const uint32_t nn = (uint32_t)big.size();
+        if (nn > sc.s_nmax) return hipErrorInvalidValue;
+        chunks.clear();
+        node_chunks.clear();
+        for (uint32_t j = 0; j < nn; j++) {
+            const int first = (int)chunks.size();
+            for (int s0 = big[j].x; s0 < big[j].y; s0 += kSahChunk)
+                chunks.push_back(make_int4((int)j, s0, std::min(big[j].y, s0 + kSahChunk), 0));
+            node_chunks.push_back(make_int2(first, (int)chunks.size() - first));
+        }
+        const uint32_t nc = (uint32_t)chunks.size();
+        if (nc > sc.s_cmax) return hipErrorInvalidValue;
+        HH_TRY(hipMemcpyAsync(sc.s_nodes, big.data(), nn * sizeof(int4), hipMemcpyHostToDevice, st));
+        HH_TRY(hipMemcpyAsync(sc.s_chunks, chunks.data(), nc * sizeof(int4), hipMemcpyHostToDevice, st));
+        HH_TRY(hipMemcpyAsync(sc.s_node_chunks, node_chunks.data(), nn * sizeof(int2), hipMemcpyHostToDevice, st));
+        k_sah_level_init<<<blocks(nn * kSahBinWords, B), B, 0, st>>>(nn, sc.s_cb, sc.s_bins);
+        k_sah_bounds<<<nc, 256, 0, st>>>(sc.s_chunks, d_order, sc.s_cen, sc.s_cb);
+        k_sah_bin<<<nc, 256, 0, st>>>(sc.s_chunks, d_order, sc.s_cen, d_lo, d_hi, sc.s_cb, sc.s_bins);
+        k_sah_choose<<<nn, 64, 0, st>>>(sc.s_nodes, sc.s_cb, sc.s_bins, splits);
+        k_sah_count<<<nc, 256, 0, st>>>(sc.s_chunks, splits, d_order, sc.s_cen, sc.s_chunk_left);
+        k_sah_chunkscan<<<blocks(nn, 64), 64, 0, st>>>(nn, sc.s_node_chunks, sc.s_chunk_left, sc.s_chunk_base);
+        k_sah_scatter<<<nc, 256, 0, st>>>(sc.s_chunks, sc.s_nodes, splits, d_order, sc.s_cen, sc.s_chunk_base, sc.s_ord2);
+        k_sah_copyback<<<nc, 256, 0, st>>>(sc.s_chunks, sc.s_ord2, d_order);
+        k_fill_u32<<<1, 64, 0, st>>>(sc.s_ctr, 0u, 1);
+        k_sah_emit<<<blocks(nn, 64), 64, 0, st>>>(nn, sc.s_nodes, splits, sc.s_meshes, childs, range, parent, sc.s_next,
+                                                 sc.s_ctr, sc.s_small, sc.s_ctr + 1);
+        HH_TRY(hipMemcpyAsync(sc.s_tail, sc.s_ctr, 8, hipMemcpyDeviceToHost, st));
+        HH_TRY(hipStreamSynchronize(st));
+        const uint32_t nnext = sc.s_tail[0];
+        if (nnext > sc.s_nmax) return hipErrorInvalidValue;
+        big.resize(nnext);
+        if (nnext) {
+            HH_TRY(hipMemcpyAsync(big.data(), sc.s_next, nnext * sizeof(int4), hipMemcpyDeviceToHost, st));
+            HH_TRY(hipStreamSynchronize(st));
+        }
+    }
+    HH_TRY(hipMemcpyAsync(sc.s_tail, sc.s_ctr + 1, 4, hipMemcpyDeviceToHost, st));
+    HH_TRY(hipStreamSynchronize(st));
+    const uint32_t nsmall = sc.s_tail[0];
+    if (nsmall > total / 2 + 1) return hipErrorInvalidValue;
+    if (nsmall)
+        k_sah_small<<<nsmall, 64, 0, st>>>(sc.s_small, sc.s_meshes, sc.s_cen, d_lo, d_hi, d_order, childs, range, parent);
+    return hipGetLastError();
 }
 
 hipError_t BuildScratch::ensure_ploc(uint32_t n) {
@@ -849,7 +1457,7 @@ void BuildScratch::release() {
 // d_node_pool + node_base (global indices in child references).
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
                       const float4* d_shade_src, float4* d_shade_pool, const float4* h_tri_src, uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
-                      float root_box[6], bool ploc) {
+                      float root_box[6], bool ploc, bool sah_gpu) {
     HH_TRY(sc.ensure(n));
     const uint32_t B = 256;
     k_init_bounds<<<1, 64, 0, st>>>(sc.cbounds);
@@ -873,13 +1481,53 @@ hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, cons
     hipError_t e;
     if (width == 8)
         e = build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base,
-                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph, ploc);
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph, ploc, sah_gpu);
     else
         e = build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base,
-                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph, ploc);
+                          tri_base, leaf_size, 0, order, sc, num_nodes, root_box, pl, ph, ploc, sah_gpu);
     HH_TRY(e);
     k_gather_tris<<<blocks(n, B), B, 0, st>>>(d_tri_src, order, n, d_tri_pool + 3 * (size_t)tri_base);
     k_gather_shade<<<blocks(n, B), B, 0, st>>>(d_shade_src, order, n, d_shade_pool + 4 * (size_t)tri_base);
+    return hipGetLastError();
+}
+
+// BLASes of several meshes over one triangle soup, by the GPU binned-SAH
+// builder: mesh g = (P, n) owns triangles P..P+n-1 of d_tri_src and of the
+// pools. The trees of all meshes are built together (one pass per level
+// over every mesh's large ranges, one launch for all small ranges); then
+// per mesh the bounds and the wide collapse, whose nodes start at
+// node_base (advancing mesh by mesh: roots[g], counts[g]); then the
+// triangles and shading records of all meshes are gathered into leaf order.
+hipError_t build_blas_forest(hipStream_t st, int width, int leaf_size, const std::vector<int2>& meshes, uint32_t total,
+                             const float4* d_tri_src, float4* d_tri_pool, const float4* d_shade_src, float4* d_shade_pool,
+                             void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* roots, uint32_t* counts,
+                             float* root_boxes) {
+    if (meshes.empty() || total == 0) return hipSuccess;
+    HH_TRY(sc.ensure(total));
+    const uint32_t B = 256;
+    k_init_bounds<<<1, 64, 0, st>>>(sc.cbounds);
+    k_tri_aabbs<<<blocks(total, B), B, 0, st>>>(d_tri_src, total, sc.aabb_lo, sc.aabb_hi, sc.cbounds);
+    HH_TRY(sah_forest_gpu(st, meshes, total, sc.aabb_lo, sc.aabb_hi, sc.order, sc.childs, sc.range, sc.parent, sc));
+    k_fill_u32<<<blocks(total, B), B, 0, st>>>(sc.flags, 0u, total);
+    for (size_t g = 0; g < meshes.size(); g++) {
+        const uint32_t P = (uint32_t)meshes[g].x, n = (uint32_t)meshes[g].y;
+        uint32_t nn = 0;
+        hipError_t e;
+        if (width == 8)
+            e = finish_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.order + P, sc.childs + P, sc.range + P, sc.parent + 2 * (size_t)P,
+                               sc.flags + P, sc.blo + 2 * (size_t)P, sc.bhi + 2 * (size_t)P, sc, leaf_size, 0,
+                               (WideNode<8>*)d_node_pool + node_base, node_base, P, &nn, root_boxes + 6 * g);
+        else
+            e = finish_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.order + P, sc.childs + P, sc.range + P, sc.parent + 2 * (size_t)P,
+                               sc.flags + P, sc.blo + 2 * (size_t)P, sc.bhi + 2 * (size_t)P, sc, leaf_size, 0,
+                               (WideNode<4>*)d_node_pool + node_base, node_base, P, &nn, root_boxes + 6 * g);
+        HH_TRY(e);
+        roots[g] = node_base;
+        counts[g] = nn;
+        node_base += nn;
+    }
+    k_gather_tris<<<blocks(total, B), B, 0, st>>>(d_tri_src, sc.order, total, d_tri_pool);
+    k_gather_shade<<<blocks(total, B), B, 0, st>>>(d_shade_src, sc.order, total, d_shade_pool);
     return hipGetLastError();
 }
 
@@ -891,7 +1539,7 @@ hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_
 
 // TLAS over instance AABBs (host arrays, n >= 1); leaves hold one instance.
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
-                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah, bool ploc) {
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah, bool ploc, bool sah_gpu) {
     HH_TRY(sc.ensure(n));
     std::vector<float4> lo(n), hi(n);
     for (uint32_t i = 0; i < n; i++) {
@@ -909,9 +1557,9 @@ hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, 
     const float* ph = sah ? h_hi : nullptr;
     if (width == 8)
         return build_wide<8>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<8>*)d_node_pool + node_base, node_base, 0, 1,
-                             1, order, sc, num_nodes, rb, pl, ph, ploc);
+                             1, order, sc, num_nodes, rb, pl, ph, ploc, sah_gpu);
     return build_wide<4>(st, n, sc.aabb_lo, sc.aabb_hi, sc.cbounds, (WideNode<4>*)d_node_pool + node_base, node_base, 0, 1, 1,
-                         order, sc, num_nodes, rb, pl, ph, ploc);
+                         order, sc, num_nodes, rb, pl, ph, ploc, sah_gpu);
 }
 
 }  // namespace hh

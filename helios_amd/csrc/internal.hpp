@@ -32,20 +32,36 @@ struct BuildScratch {
     int2* p_child = nullptr;
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
+    // binned SAH on the GPU, allocated on first use
+    uint32_t s_cap = 0, s_nmax = 0, s_cmax = 0;
+    float4* s_cen = nullptr;
+    uint32_t *s_ord2 = nullptr, *s_cb = nullptr, *s_bins = nullptr, *s_chunk_left = nullptr, *s_chunk_base = nullptr;
+    uint32_t *s_ctr = nullptr, *s_tail = nullptr;   // s_tail: pinned host
+    int4 *s_nodes = nullptr, *s_next = nullptr, *s_small = nullptr, *s_chunks = nullptr;
+    int2 *s_node_chunks = nullptr, *s_meshes = nullptr;
+    uint32_t s_mesh_cap = 0;
+    void* s_splits = nullptr;
     hipError_t ensure(uint32_t n);
     hipError_t ensure_ploc(uint32_t n);
+    hipError_t ensure_sah(uint32_t n);
+    hipError_t ensure_sah_meshes(uint32_t nm);
     void release();
     void release_ploc();
+    void release_sah();
     uint32_t* vals_out_tmp(uint32_t) { return order; }
-    ~BuildScratch() { release(); release_ploc(); }
+    ~BuildScratch() { release(); release_ploc(); release_sah(); }
 };
 
 hipError_t build_blas(hipStream_t st, int width, int leaf_size, uint32_t n, const float4* d_tri_src, float4* d_tri_pool,
                       const float4* d_shade_src, float4* d_shade_pool, const float4* h_tri_src,
                       uint32_t tri_base, void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes,
-                      float root_box[6], bool ploc);
+                      float root_box[6], bool ploc, bool sah_gpu);
 hipError_t build_tlas(hipStream_t st, int width, uint32_t n, const float* h_lo, const float* h_hi, void* d_node_pool,
-                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah, bool ploc);
+                      uint32_t node_base, BuildScratch& sc, uint32_t* num_nodes, bool sah, bool ploc, bool sah_gpu);
+hipError_t build_blas_forest(hipStream_t st, int width, int leaf_size, const std::vector<int2>& meshes, uint32_t total,
+                             const float4* d_tri_src, float4* d_tri_pool, const float4* d_shade_src, float4* d_shade_pool,
+                             void* d_node_pool, uint32_t node_base, BuildScratch& sc, uint32_t* roots, uint32_t* counts,
+                             float* root_boxes);
 hipError_t quantize_bvh4(hipStream_t st, const void* d_src, void* d_dst, uint32_t n);
 
 // Ray queues are split into kQShards segments, each with its own length and
@@ -126,7 +142,7 @@ struct Tuning {
     int leaf_size = 4;
     uint32_t frames_in_flight = 4;
     bool bvh_compressed = false;
-    uint32_t bvh_builder = 1;   // 0: LBVH on the GPU; 1: binned SAH on the host; 2: PLOC on the GPU
+    uint32_t bvh_builder = 3;   // 0: LBVH on the GPU; 1: binned SAH on the host; 2: PLOC on the GPU; 3: binned SAH on the GPU
     uint32_t refill_lanes = 24;  // idle lanes that make a traversal wave take new rays
     int traversal = 0;           // 0 while-while, 1 if-if step shape
     uint32_t continuation_depth = 0;  // bounces before the continuation launch (0 = none)

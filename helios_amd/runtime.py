@@ -157,7 +157,7 @@ class Context:
         check(lib().hh_sync(self.h))
 
     def set_tuning(self, bvh_width=4, extend_block=256, queue_capacity=0, persistent_blocks_per_cu=8,
-                   frames_in_flight=4, bvh_compressed=False, refill_lanes=24, traversal=0, bvh_builder=1,
+                   frames_in_flight=4, bvh_compressed=False, refill_lanes=24, traversal=0, bvh_builder=3,
                    continuation_depth=0):
         t = abi.Tuning(bvh_width, extend_block, extend_block, queue_capacity, persistent_blocks_per_cu,
                        frames_in_flight, int(bool(bvh_compressed)), refill_lanes, traversal, bvh_builder,

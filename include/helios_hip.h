@@ -248,7 +248,8 @@ hh_status hh_scene_add_texture(hh_scene scene, uint32_t width, uint32_t height, 
                                const void* texels, uint32_t* out_id);
 hh_status hh_scene_add_material(hh_scene scene, const hh_material_desc* desc, uint32_t* out_id);
 /* Mesh::create (mesh.cpp:27-42): vertices/indices/submeshes + the mesh's
- * material list (material ids). */
+ * material list (material ids). Vertex positions must be finite
+ * (HH_ERR_INVALID_ARG otherwise). */
 hh_status hh_scene_add_mesh(hh_scene scene, const hh_vertex* vertices, uint32_t num_vertices,
                             const uint32_t* indices, uint32_t num_indices,
                             const hh_submesh* submeshes, uint32_t num_submeshes,
@@ -525,11 +526,13 @@ typedef struct hh_tuning {
      * the lane holds a leaf, then the leaf), 1 = if-if (one node or leaf per
      * step). Results are identical; only the schedule differs. */
     uint32_t traversal;
-    /* BLAS / TLAS builder, used at the next scene commit: 0 = LBVH on the GPU
-     * (Morton order, fast build), 1 = binned SAH on the host (fast trace,
-     * the role of PREFER_FAST_TRACE in mesh.cpp:109; the default), 2 = PLOC
-     * clustering on the GPU (Meister and Bittner 2018; radius 16). Hits are
-     * identical either way (ties resolve by instance, geometry, primitive). */
+    /* BLAS / TLAS builder, used at the next scene commit: 3 = binned SAH on
+     * the GPU, every mesh's tree in one build (fast trace, the role of
+     * PREFER_FAST_TRACE in mesh.cpp:109; the default); 1 = the same binned
+     * SAH on the host (the same splits, hence the same tree); 0 = LBVH on the
+     * GPU (Morton order); 2 = PLOC clustering on the GPU (Meister and Bittner
+     * 2018; radius 16). Hits are identical either way (ties resolve by
+     * instance, geometry, primitive). */
     uint32_t bvh_builder;
     /* Paths still alive after this many bounces traced as wavefront launches
      * finish in one continuation launch, in which a lane whose path ends takes

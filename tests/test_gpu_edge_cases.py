@@ -8,7 +8,8 @@
     FP32 transcription (relMSE < 1e-4, the bar of the main parity tests);
   * a one-bounce integrator (max_ray_bounces 1: emission + NEE only);
   * the reference's instance limit (scene.h:14, 1024): 1024 instances commit,
-    the 1025th is refused with HH_ERR_LIMIT_EXCEEDED.
+    the 1025th is refused with HH_ERR_LIMIT_EXCEEDED;
+  * a non-finite vertex position is refused with HH_ERR_INVALID_ARG.
 """
 import numpy as np
 import pytest
@@ -128,6 +129,23 @@ def test_instance_limit(gpu_ctx):
         d.instances.append(d.instances[0])
         extra.load(d)
     assert e.value.status == 3
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_non_finite_vertex_position_is_refused(gpu_ctx, bad):
+    """A NaN or infinite vertex position is refused at hh_scene_add_mesh
+    (HH_ERR_INVALID_ARG) before any BVH build sees it."""
+    from helios_amd import runtime
+    from helios_amd.runtime import HeliosError
+    d = scenegen.SceneDesc(width=8, height=8)
+    d.materials.append(scenegen.material())
+    V = np.array([[0.0, 0.0, 0.0], [1.0, bad, 0.0], [0.0, 1.0, 0.0]])
+    d.meshes.append(scenegen._mesh_from_parts([(V, np.tile([0.0, 0.0, 1.0], (3, 1)), None,
+                                                np.array([0, 1, 2], np.uint32), 0)], [0]))
+    d.instances.append(scenegen.identity_instance(0))
+    with pytest.raises(HeliosError) as e:
+        runtime.Scene(gpu_ctx, d)
+    assert e.value.status == 1
 
 
 def _quad(x0, x1, y0, y1, z):

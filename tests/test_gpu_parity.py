@@ -230,12 +230,13 @@ def test_progressive_protocol(gpu_ctx, cornell, cornell_pair):
 @pytest.mark.parametrize("layout", [dict(bvh_width=8), dict(bvh_width=4, bvh_compressed=True),
                                     dict(bvh_builder=0), dict(bvh_builder=0, bvh_width=8),
                                     dict(bvh_builder=2), dict(bvh_builder=2, bvh_width=8),
-                                    dict(bvh_builder=2, bvh_compressed=True)])
+                                    dict(bvh_builder=2, bvh_compressed=True),
+                                    dict(bvh_builder=3), dict(bvh_builder=3, bvh_width=8)])
 def test_bvh_layouts_bit_identical(gpu_ctx, small_field, field_pair, oracle_mod, layout):
-    """BVH8, the quantized BVH4, the LBVH and the clustered (PLOC) trees
-    return the same hits as the oracle and the same images as the binned-SAH
-    float BVH4 (traversal results are independent of the tree's shape and
-    visit order)."""
+    """BVH8, the quantized BVH4, the LBVH, the clustered (PLOC) and the
+    GPU-built binned-SAH trees return the same hits as the oracle and the same
+    images as the host-built binned-SAH float BVH4 (traversal results are
+    independent of the tree's shape and visit order)."""
     from helios_amd import runtime
     g4, o = field_pair
     ref_img, _ = _render_gpu(gpu_ctx, g4, small_field, frames=3)
@@ -253,6 +254,37 @@ def test_bvh_layouts_bit_identical(gpu_ctx, small_field, field_pair, oracle_mod,
         g.close()
     finally:
         gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("level,width", [(3, 4), (5, 4), (5, 8)])
+def test_gpu_sah_builder_builds_the_host_tree(gpu_ctx, level, width):
+    """The GPU binned-SAH builder (bvh_builder 3) makes the host builder's
+    splits: same node counts, and the counting traversal fetches exactly the
+    same nodes, triangles and instances for every ray of three frames, at
+    mesh sizes that use the block-wide level passes (level 5: 20,480
+    triangles per sphere) and the one-wave subtree builds."""
+    from helios_amd import runtime
+    d = scenegen.icosphere_field(seed=4, n_spheres=5, level=level, n_leaves=300, width=96, height=64,
+                                 max_ray_bounces=4)
+    out = {}
+    try:
+        for b in (1, 3):
+            gpu_ctx.set_tuning(bvh_builder=b, bvh_width=width)
+            sc = runtime.Scene(gpu_ctx, d)
+            gpu_ctx.set_profiling(timing=True, counting=True)
+            gpu_ctx.reset_profile()
+            img, stats = _render_gpu(gpu_ctx, sc, d, frames=3)
+            counts = gpu_ctx.read_profile()["counts"]
+            gpu_ctx.set_profiling()
+            out[b] = (sc.bvh_stats(), counts, img, stats)
+            sc.close()
+    finally:
+        gpu_ctx.set_profiling()
+        gpu_ctx.set_tuning()
+    assert out[1][0] == out[3][0]
+    assert out[1][1] == out[3][1] and out[1][1]["ext_nodes"] > 0
+    assert np.array_equal(out[1][2], out[3][2])
+    assert out[1][3] == out[3][3]
 
 
 @pytest.mark.parametrize("fif", [4, 8, 64])
