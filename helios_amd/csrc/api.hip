@@ -1049,9 +1049,17 @@ hh_status hh_scene_commit(hh_scene s) {
     }
     uint32_t tlas_root = node_base, tlas_nodes = 0;
     if (s->instances.empty()) {
+        // a root without children: all-infinite child boxes, which every ray misses
         std::vector<char> empty(node_bytes, 0);
+        auto fill_empty = [&](auto* nd, int w) {
+            for (int k = 0; k < w; k++)
+                nd->lox[k] = nd->loy[k] = nd->loz[k] = nd->hix[k] = nd->hiy[k] = nd->hiz[k] = INFINITY;
+        };
+        if (W == 8) fill_empty((WideNode<8>*)empty.data(), 8);
+        else fill_empty((WideNode<4>*)empty.data(), 4);
         HH_HIP(hipMemcpyAsync((char*)d_nodes + size_t(node_base) * node_bytes, empty.data(), node_bytes, hipMemcpyHostToDevice,
                               ctx->stream));
+        HH_HIP(hipStreamSynchronize(ctx->stream));   // `empty` goes out of scope
         tlas_nodes = 1;
     } else {
         hipError_t e = build_tlas(ctx->stream, W, (uint32_t)s->instances.size(), ilo.data(), ihi.data(), d_nodes, node_base, ctx->bs,

@@ -250,10 +250,12 @@ __global__ void k_collapse(const int2* __restrict__ jobs, const uint32_t* __rest
             c[nc++] = childs[r].y;
         }
     }
+    // absent children get all-infinite boxes, which every ray misses (the
+    // traversal's box test needs no child count)
     WideNode<W> nd;
     for (int k = 0; k < W; k++) {
-        nd.lox[k] = nd.loy[k] = nd.loz[k] = 0.f;
-        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.f;
+        nd.lox[k] = nd.loy[k] = nd.loz[k] = INFINITY;
+        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = INFINITY;
         nd.child[k] = 0;
     }
     nd.nchild = (uint32_t)nc;

@@ -219,12 +219,11 @@ __device__ inline void node_test(const HH_GLOBAL void* nodes, uint32_t cur, f3 o
         }
     } else {
         constexpr int W = layout_width(L);
+        // absent children have all-infinite boxes, which every ray misses
         const HH_GLOBAL WideNode<W>& nd = ((const HH_GLOBAL WideNode<W>*)nodes)[cur];
-        const uint32_t nc = nd.nchild;
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            float t = slab(nd.lox[k], nd.hix[k], nd.loy[k], nd.hiy[k], nd.loz[k], nd.hiz[k], o, inv, tmin, tmax);
-            tn[k] = (uint32_t)k < nc ? t : INFINITY;
+            tn[k] = slab(nd.lox[k], nd.hix[k], nd.loy[k], nd.hiy[k], nd.loz[k], nd.hiz[k], o, inv, tmin, tmax);
             ref[k] = nd.child[k];
         }
     }
