@@ -337,14 +337,24 @@ struct Tracer {
             pop(stk);
             return;
         }
+        auto cswap = [&](int i, int j) {   // selects, not branches
+            const bool c = tn[j] < tn[i];
+            const float a = tn[i], b = tn[j];
+            const uint32_t ra = ref[i], rb = ref[j];
+            tn[i] = c ? b : a;
+            tn[j] = c ? a : b;
+            ref[i] = c ? rb : ra;
+            ref[j] = c ? ra : rb;
+        };
+        if constexpr (W == 4) {
+            // the 5-comparator sorting network for 4 keys
+            cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+        } else {
 #pragma unroll
-        for (int i = 1; i < W; i++)
+            for (int i = 1; i < W; i++)
 #pragma unroll
-            for (int j = W - 1; j >= i; j--)
-                if (tn[j] < tn[j - 1]) {
-                    float tt = tn[j]; tn[j] = tn[j - 1]; tn[j - 1] = tt;
-                    uint32_t rr = ref[j]; ref[j] = ref[j - 1]; ref[j - 1] = rr;
-                }
+                for (int j = W - 1; j >= i; j--) cswap(j - 1, j);
+        }
         if (tn[0] == INFINITY) {
             pop(stk);
             return;
