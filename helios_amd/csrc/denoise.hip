@@ -74,6 +74,14 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
     constexpr int TILE = TR * TC * 4;
     constexpr int SPT = (TILE + kConvThreads - 1) / kConvThreads;  // staged 16-B chunks per thread
     __shared__ bf16x8 tile[TILE];            // [row][col][4 chunks of 8 channels]
+    // Up to 48 output channels, the group's weights are staged in LDS too
+    // ([tap][co] rows of 32 channels + 8 of padding, so the 16 rows a wave
+    // reads at once fall on distinct banks), loaded with the tile.
+    constexpr bool LDSW = COUT <= 96;
+    constexpr int WROW = 5;                  // 16-B chunks per (tap, co) row
+    constexpr int WCH = 9 * COUT * 4;        // weight chunks per group
+    constexpr int WPT = LDSW ? (WCH + kConvThreads - 1) / kConvThreads : 1;
+    __shared__ bf16x8 wl[LDSW ? 9 * COUT * WROW : 1];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t r = lane & 15u, g = lane >> 4;
     const uint32_t segs = (W + TW - 1) / TW, rblocks = (H + kConvRows - 1) / kConvRows;
@@ -94,7 +102,20 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
     // issued before any is stored, and before the barrier that waits for the
     // previous group's MFMAs (the staging registers are dead during them).
     bf16x8 stage[SPT];
+    bf16x8 wstage[WPT];
     auto load_group = [&](int cg) {
+        if constexpr (LDSW) {
+#pragma unroll
+            for (int i = 0; i < WPT; i++) {
+                const int e = (int)threadIdx.x + i * kConvThreads;
+                const int j = e & 3, row = e >> 2;     // 8-channel chunk of row (tap, co)
+                const int co = row % COUT, tap = row / COUT;
+                const int c = cg * 32 + j * 8;
+                bf16x8 v = zero8();
+                if (e < WCH && c < CT) v = *(const bf16x8*)(wgt + (size_t)co * KPAD + tap * CT + c);
+                wstage[i] = v;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < SPT; i++) {
             const int e = (int)threadIdx.x + i * kConvThreads;
@@ -122,6 +143,13 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
             const int e = (int)threadIdx.x + i * kConvThreads;
             if (e < TILE) tile[e] = stage[i];
         }
+        if constexpr (LDSW) {
+#pragma unroll
+            for (int i = 0; i < WPT; i++) {
+                const int e = (int)threadIdx.x + i * kConvThreads;
+                if (e < WCH) wl[(e >> 2) * WROW + (e & 3)] = wstage[i];
+            }
+        }
         __syncthreads();
         const int c = cg * 32 + (int)g * 8;      // this lane's 8 channels
         const bool cok = c < CT;
@@ -134,7 +162,9 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
             const int k = tap * CT + c;
 #pragma unroll
             for (int m = 0; m < MT; m++) {
-                const bf16x8 a = cok ? *(const bf16x8*)(wgt + (size_t)(m * 16 + r) * KPAD + k) : zero8();
+                bf16x8 a;
+                if constexpr (LDSW) a = wl[(tap * COUT + m * 16 + (int)r) * WROW + (int)g];
+                else a = cok ? *(const bf16x8*)(wgt + (size_t)(m * 16 + r) * KPAD + k) : zero8();
 #pragma unroll
                 for (int t = 0; t < NT; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[m][t], 0, 0, 0);
             }
