@@ -71,6 +71,8 @@ struct GraphEntry {
 struct hh_context_t {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                       // shadow launches beside the next extension launch
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int num_cu = 0;
     Tuning tune;
     Wavefront wf;
@@ -387,6 +389,9 @@ hh_status hh_context_create(int device_ordinal, hh_context* out) {
     if (!c) return fail(HH_ERR_OUT_OF_MEMORY, "context");
     c->device = device_ordinal;
     HH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HH_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HH_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HH_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HH_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device_ordinal));
     HH_HIP(hipMalloc(&c->d_coef, 30 * sizeof(float)));
     *out = c;
@@ -407,6 +412,10 @@ hh_status hh_context_destroy(hh_context ctx) {
     if (ctx->prof.d_counts) (void)hipFree(ctx->prof.d_counts);
     if (ctx->prof.d_snaps) (void)hipFree(ctx->prof.d_snaps);
     for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(ctx->side);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HH_OK;
@@ -1467,11 +1476,22 @@ static hh_status run_frame(hh_context ctx, const std::vector<SceneGPU>& scenes, 
     uint32_t* spill_before = ctx->wf.spill;
     const uint64_t qcap_before = ctx->wf.qcap;
     HH_HIP(ensure_wavefront(ctx, total));
-    HH_HIP(ctx->wf.ensure_spill((size_t)ctx->num_cu * ctx->tune.blocks_per_cu * ctx->tune.extend_block));
+    // Spill area for the most threads a frame launches at once: twice the
+    // traversal grid (the shadow launch on the side stream has its own half),
+    // and the counting frames' 256-thread blocks. launch_frame runs blocks of
+    // 64 or 128 threads when extend_block says so, else 256.
+    {
+        const int eb = ctx->tune.extend_block, bs = (eb == 64 || eb == 128) ? eb : 256;
+        const size_t per_block = std::max<size_t>(2 * (size_t)bs, 256);
+        HH_HIP(ctx->wf.ensure_spill((size_t)ctx->num_cu * ctx->tune.blocks_per_cu * per_block));
+    }
     if (ctx->wf.capacity != cap_before || ctx->wf.qcap != qcap_before || ctx->wf.spill != spill_before) ctx->drop_graphs();
     HH_HIP(hipMemcpyAsync(ctx->d_scene_table, scenes.data(), scenes.size() * sizeof(SceneGPU), hipMemcpyHostToDevice, ctx->stream));
     HH_HIP(hipMemcpyAsync(ctx->d_views, views.data(), views.size() * sizeof(ViewGPU), hipMemcpyHostToDevice, ctx->stream));
     LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
+    cfg.side = ctx->side;
+    cfg.fork = ctx->ev_fork;
+    cfg.join = ctx->ev_join;
     if (ctx->prof.timing || ctx->prof.count) {
         // profiled frames are launched directly so every kernel gets its events
         cfg.prof = &ctx->prof;

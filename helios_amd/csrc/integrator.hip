@@ -1355,6 +1355,17 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     bool packets = !COUNT;
     for (uint32_t i = 0; i < nv; i++)
         packets = packets && h_views[i].scene == h_views[0].scene && h_views[i].spp >= kPacketMinSpp;
+    // Shadow launches on the side stream, beside the next extension launch
+    // (neither reads what the other writes); `join` makes the main stream
+    // wait for the last one before the next reset, shade, continuation or
+    // accumulation. Profiled frames stay on one stream.
+    const bool side = pf == nullptr && cfg.side != nullptr && wf.spill_threads >= 2ull * grid * BS;
+    uint32_t* const spill_side = side ? wf.spill + (size_t)grid * BS * kSpillStackDepth : wf.spill;
+    bool joined = true;
+    auto join = [&]() {
+        if (!joined) (void)hipStreamWaitEvent(st, cfg.join, 0);
+        joined = true;
+    };
     // one bounce of every path in the queue: extension rays, shading, shadow rays
     auto bounce = [&](uint32_t d) {
         const int a = d & 1, b = a ^ 1;
@@ -1372,6 +1383,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
                                                               d == 0 ? 0u : (uint32_t)HH_TRACE_OPAQUE_BIT, wf.hit,
                                                               wf.hit_tri, wf.stats, tc, wf.spill);
         }
+        join();
         k_reset_queues<<<1, 64, 0, st>>>(ctr, kCtrExt0 + b);
         {
             ProfScope ps(pf, PK_SHADE, st);
@@ -1381,12 +1393,22 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
                                              ctr + ctr_index(kCtrExt0 + b, 0), wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
                                              ctr + ctr_index(kCtrShadow, 0), wf.feat, wf.capacity);
         }
+        hipStream_t sst = st;
+        if (side) {
+            (void)hipEventRecord(cfg.fork, st);
+            (void)hipStreamWaitEvent(cfg.side, cfg.fork, 0);
+            sst = cfg.side;
+        }
         {
-            ProfScope ps(pf, PK_SHADOW, st);
-            k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
-                                                              ctr + ctr_index(kCtrShadow, 0),
-                                                              ctr + ctr_index(kCtrShadowHead, 0), seg, refill, wf.L,
-                                                              wf.stats, tc, wf.spill);
+            ProfScope ps(pf, PK_SHADOW, sst);
+            k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, sst>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+                                                               ctr + ctr_index(kCtrShadow, 0),
+                                                               ctr + ctr_index(kCtrShadowHead, 0), seg, refill, wf.L,
+                                                               wf.stats, tc, side ? spill_side : wf.spill);
+        }
+        if (side) {
+            (void)hipEventRecord(cfg.join, cfg.side);
+            joined = false;
         }
     };
     {
@@ -1403,6 +1425,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     const uint32_t cont = COUNT ? 0u : (mega ? 1u : cfg.tune.continuation_depth);
     if (cont > 0 && cont < max_bounces) {
         for (uint32_t d = 0; d < cont; d++) bounce(d);
+        join();
         const int q = (int)(cont & 1u);  // bounce(cont - 1) queued its extension rays there
         ProfScope ps(pf, PK_MEGAKERNEL, st);
         k_megakernel<W, BS><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[q], wf.q_d[q], wf.q_slot[q],
@@ -1410,6 +1433,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
                                                  refill_paths, wf.T, wf.rng, wf.L, wf.stats, wf.spill, cont);
     } else {
         for (uint32_t d = 0; d < max_bounces; d++) bounce(d);
+        join();
     }
     uint32_t maxpix = 0;
     for (uint32_t i = 0; i < nv; i++) maxpix = std::max(maxpix, h_views[i].tile_w * h_views[i].tile_h);

@@ -190,6 +190,12 @@ struct LaunchCfg {
     int num_cu;
     Tuning tune;
     Prof* prof = nullptr;
+    // A second stream and two events: when set (and not profiling), each
+    // bounce's shadow launch runs on `side` beside the next bounce's
+    // extension launch, joined before anything else touches the shadow queue
+    // or the radiance. Needs a spill area for twice the launched threads.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 hipError_t launch_frame(const LaunchCfg& cfg, const SceneGPU* d_scenes, const SceneGPU* h_scenes, const ViewGPU* d_views,
                         const ViewGPU* h_views, uint32_t num_views, uint32_t total_paths, uint32_t max_bounces,
