@@ -952,12 +952,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
             } else {
                 Surface p;
                 populate(S, h.inst, h.tri, h.u, h.v, p);
-                ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, V.flags, rng);
-                if (sr.want) {
-                    HitRec sh;
-                    n_sh++;
-                    if (!trace_ray<W, true>(S, sr.o, sr.d, sr.tmin, sr.tmax, sr.flags, sh, stk)) Lp = Lp + sr.contrib;
-                }
+                const ShadowRay sr = direct_lighting(S, p, d, T, depth, V.shadow_bias, V.flags, rng);
+                // The bounce is sampled before the shadow ray is traced (the
+                // RNG draws keep their order: light sample, BRDF sample, RR),
+                // so the surface is dead across that traversal.
                 if ((V.flags & HH_FLAG_DIRECT_ONLY_BIT) != 0u || depth + 1 >= V.max_bounces) {
                     finish = true;
                 } else {
@@ -978,6 +976,11 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
                         rflags = HH_TRACE_OPAQUE_BIT;
                         depth++;
                     }
+                }
+                if (sr.want) {
+                    HitRec sh;
+                    n_sh++;
+                    if (!trace_ray<W, true>(S, sr.o, sr.d, sr.tmin, sr.tmax, sr.flags, sh, stk)) Lp = Lp + sr.contrib;
                 }
             }
             if (finish) {

@@ -298,18 +298,21 @@ struct Tracer {
         cur = scene.tlas_root;
     }
 
-    // A push beyond the stack capacity (never written out of bounds) ends
-    // the ray: `trips` takes kStackFull, which stops the visit loop, and the
-    // scene's fault counter records it; the capacity is far above the depth
-    // of the built trees.
+    // A push beyond the stack capacity is not stored (never written out of
+    // bounds) but still counted in sp; check_full, called once after a
+    // visit's pushes and before any pop, then ends the ray: `trips` takes
+    // kStackFull, which stops the visit loop, and the scene's fault counter
+    // records it. The capacity is far above the depth of the built trees.
     __device__ inline void push(const LaneStack& stk, uint32_t r) {
         if (sp < (uint32_t)kLdsStack) stk.lds[sp * stk.lds_stride] = r;
         else if (sp < (uint32_t)(kLdsStack + kSpillStack)) stk.spill[(sp - kLdsStack) * stk.spill_stride] = r;
-        else {
-            trips = kStackFull;
-            return;
-        }
         sp++;
+    }
+    __device__ inline void check_full() {
+        if (sp > (uint32_t)(kLdsStack + kSpillStack)) {
+            trips = kStackFull;
+            sp = kLdsStack + kSpillStack;
+        }
     }
     __device__ inline void pop(const LaneStack& stk) {
         if (sp == 0) {
@@ -334,6 +337,7 @@ struct Tracer {
 #pragma unroll
             for (int k = W - 1; k >= 0; k--)
                 if (tn[k] != INFINITY) push(stk, ref[k]);
+            check_full();
             pop(stk);
             return;
         }
@@ -365,6 +369,7 @@ struct Tracer {
 #pragma unroll
         for (int k = W - 1; k >= 1; k--)
             if (tn[k] != INFINITY) push(stk, ref[k]);
+        check_full();
         cur = ref[0];
     }
 
@@ -432,6 +437,7 @@ struct Tracer {
             inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
             wt = wt_pre(od);
             push(stk, kMarker);
+            check_full();
             cur = it.blas_root;
             return false;
         } else {
