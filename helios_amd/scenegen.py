@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import VERTEX_DTYPE, LIGHT_DIRECTIONAL, LIGHT_POINT, LIGHT_SPOT, TEXTURE_RGBA8_SRGB
+from .abi import VERTEX_DTYPE, LIGHT_DIRECTIONAL, LIGHT_POINT, LIGHT_SPOT, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM
 
 
 @dataclass
@@ -656,6 +656,43 @@ def _png_bytes(rgba8):
             chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
 
 
+def _srgb_decode(b):
+    """sRGB byte -> linear value, as the glTF loader decodes it."""
+    c = np.asarray(b, np.float64) / 255.0
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+def _export_emissive_texture(tex, mat, add_png_texture):
+    """Emission texture -> (emissiveTexture, emissiveFactor, emissive
+    strength) in `mat`; returns the factor x strength triple. glTF stores it
+    as sRGB bytes scaled by a per-channel factor, so an RGBA32F texture is
+    exported with strength = its largest texel, factor = each channel's
+    largest / strength, and bytes = the sRGB encoding of texel / (factor x
+    strength); it is refused when the loader's decode of those bytes misses a
+    texel by more than 1e-6 relative (values quantized in between)."""
+    from .abi import TEXTURE_RGBA32F
+    w, h, fmt, data = tex
+    if fmt == TEXTURE_RGBA8_SRGB:
+        mat["emissiveTexture"] = {"index": add_png_texture(np.asarray(data, np.uint8).reshape(h, w, 4))}
+        return [1.0, 1.0, 1.0]
+    if fmt != TEXTURE_RGBA32F:
+        raise NotImplementedError(f"emissive textures of format {fmt}")
+    t = np.asarray(data, np.float32).reshape(h, w, 4).astype(np.float64)
+    peak = t[..., :3].max(axis=(0, 1))
+    if peak.max() <= 0.0:
+        return [0.0, 0.0, 0.0]
+    scale = np.where(peak > 0.0, peak, 1.0)
+    lin = np.clip(t[..., :3] / scale, 0.0, 1.0)
+    enc = np.where(lin <= 0.0031308, lin * 12.92, 1.055 * lin ** (1.0 / 2.4) - 0.055)
+    b = np.clip(np.round(enc * 255.0), 0, 255).astype(np.uint8)
+    back = (_srgb_decode(b) * peak).astype(np.float32)
+    if not np.allclose(back, t[..., :3], rtol=1e-6, atol=1e-7):
+        raise NotImplementedError("emissive texture not representable as sRGB bytes x a factor")
+    rgba = np.concatenate([b, np.full((h, w, 1), 255, np.uint8)], axis=-1)
+    mat["emissiveTexture"] = {"index": add_png_texture(rgba)}
+    return [float(x) for x in peak]
+
+
 def _quat_looking_along(fwd):
     """(x, y, z, w) of a rotation taking -Z to fwd (glTF lights point along -Z)."""
     f = np.asarray(fwd, np.float64)
@@ -675,10 +712,15 @@ def write_gltf(desc, path):
     their model matrix, materials -> metallic-roughness materials (linear
     baseColorFactor = the packed albedo pow(v, 2.2); emission above 1 through
     KHR_materials_emissive_strength; alpha-tested -> MASK at cutoff 0.1,
-    transparent -> BLEND), sRGB RGBA8 albedo textures -> embedded PNGs, the
-    camera -> a camera node, lights -> KHR_lights_punctual (their radius is
-    not representable). Material overrides, IBL cube maps, normal / roughness
-    / metallic / emissive textures and the sky are not exported."""
+    transparent -> BLEND), sRGB RGBA8 albedo textures -> embedded PNGs,
+    RGBA8 UNORM normal textures -> normalTexture, RGBA8 UNORM roughness and
+    metallic textures -> one metallicRoughnessTexture (their channels copied
+    to G and B), emissive textures -> emissiveTexture x emissiveFactor x
+    KHR_materials_emissive_strength when that reproduces every texel (see
+    _export_emissive_texture), the camera -> a camera node, lights ->
+    KHR_lights_punctual (their radius is not representable). Material
+    overrides, albedo textures other than sRGB RGBA8, IBL cube maps and the
+    sky are not exported."""
     import json
     import struct
     from pathlib import Path
@@ -707,19 +749,47 @@ def write_gltf(desc, path):
         return len(accessors) - 1
 
     views, accessors, images, textures, materials, meshes, nodes = [], [], [], [], [], [], []
+
+    def add_png_texture(rgba8):
+        images.append({"bufferView": add_view(_png_bytes(rgba8)), "mimeType": "image/png"})
+        textures.append({"source": len(images) - 1})
+        return len(textures) - 1
+
+    def tex_rgba8(ti, fmt_wanted, what):
+        w, h, fmt, data = desc.textures[ti]
+        if fmt != fmt_wanted:
+            raise NotImplementedError(f"{what} textures of format {fmt}")
+        return np.asarray(data, np.uint8).reshape(h, w, 4)
+
     tex_index = {}
     for ti, (w, h, fmt, data) in enumerate(desc.textures):
-        if fmt != TEXTURE_RGBA8_SRGB:
-            continue
-        images.append({"bufferView": add_view(_png_bytes(np.asarray(data, np.uint8).reshape(h, w, 4))),
-                       "mimeType": "image/png"})
-        textures.append({"source": len(images) - 1})
-        tex_index[ti] = len(textures) - 1
+        if fmt == TEXTURE_RGBA8_SRGB:
+            tex_index[ti] = add_png_texture(np.asarray(data, np.uint8).reshape(h, w, 4))
     for m in desc.materials:
-        for k in ("normal_texture", "roughness_texture", "metallic_texture", "emissive_texture"):
-            if m.get(k, -1) != -1:
-                raise NotImplementedError(f"export of {k}")
         pbr = {"metallicFactor": float(m["metallic_value"]), "roughnessFactor": float(m["roughness_value"])}
+        mat = {}
+        rt, mt = m.get("roughness_texture", -1), m.get("metallic_texture", -1)
+        if rt != -1 or mt != -1:
+            # glTF keeps roughness in G and metallic in B of one texture; the
+            # loader reads them back through channels 1 and 2, so each
+            # channel's bytes (or 255 x the constant as factor) are copied
+            rough = tex_rgba8(rt, TEXTURE_RGBA8_UNORM, "roughness") if rt != -1 else None
+            metal = tex_rgba8(mt, TEXTURE_RGBA8_UNORM, "metallic") if mt != -1 else None
+            shape = (rough if rough is not None else metal).shape
+            if rough is not None and metal is not None and rough.shape != metal.shape:
+                raise NotImplementedError("roughness and metallic textures of different sizes")
+            mr = np.zeros(shape, np.uint8)
+            mr[..., 3] = 255
+            mr[..., 1] = rough[..., int(m["roughness_channel"])] if rough is not None else 255
+            mr[..., 2] = metal[..., int(m["metallic_channel"])] if metal is not None else 255
+            pbr["metallicRoughnessTexture"] = {"index": add_png_texture(mr)}
+            if rough is not None:
+                pbr["roughnessFactor"] = 1.0
+            if metal is not None:
+                pbr["metallicFactor"] = 1.0
+        if m.get("normal_texture", -1) != -1:
+            mat["normalTexture"] = {"index": add_png_texture(tex_rgba8(m["normal_texture"], TEXTURE_RGBA8_UNORM,
+                                                                        "normal"))}
         alb = m["albedo_value"]
         if m.get("albedo_texture", -1) != -1:
             if m["albedo_texture"] not in tex_index:
@@ -727,8 +797,10 @@ def write_gltf(desc, path):
             pbr["baseColorTexture"] = {"index": tex_index[m["albedo_texture"]]}
         else:
             pbr["baseColorFactor"] = [float(np.float32(alb[k]) ** np.float32(2.2)) for k in range(3)] + [float(alb[3])]
-        mat = {"pbrMetallicRoughness": pbr}
+        mat["pbrMetallicRoughness"] = pbr
         em = [float(x) for x in m["emissive_value"][:3]]
+        if m.get("emissive_texture", -1) != -1:
+            em = _export_emissive_texture(desc.textures[m["emissive_texture"]], mat, add_png_texture)
         if max(em) > 0.0:
             peak = max(em)
             if peak > 1.0:

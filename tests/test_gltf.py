@@ -295,6 +295,55 @@ def test_exported_field_glb_renders_the_same(oracle_mod, tmp_path):
     np.testing.assert_allclose(a.render(frames=2), b.render(frames=2), atol=5e-5)
 
 
+def test_exported_textured_materials_render_the_same(oracle_mod, tmp_path):
+    """Normal textures, roughness and metallic textures (one
+    metallicRoughnessTexture, G and B) and a float emission texture (sRGB
+    bytes x emissiveFactor x emissive strength) survive export and load: the
+    same hits, every texel the shader reads equal, and the same radiance (the
+    lights' radius set to 0: glTF lights are punctual)."""
+    d = scenegen.textured_showcase(width=48, height=32)
+    for l in d.lights:
+        l["radius"] = 0.0
+    scenegen.write_gltf(d, tmp_path / "tex.gltf")
+    d2, info = runtime.gltf_scene_desc(tmp_path / "tex.gltf", width=d.width, height=d.height,
+                                       max_ray_bounces=d.max_ray_bounces)
+    m0, m1 = d.materials[0], d2.materials[0]
+
+    def texels(desc, ti):
+        w, h, fmt, data = desc.textures[ti]
+        return fmt, np.asarray(data).reshape(h, w, 4)
+
+    # what the shader reads: albedo bytes, normal bytes, roughness / metallic channels, emission
+    assert np.array_equal(texels(d, m0["albedo_texture"])[1], texels(d2, m1["albedo_texture"])[1])
+    assert np.array_equal(texels(d, m0["normal_texture"])[1], texels(d2, m1["normal_texture"])[1])
+    for k in ("roughness", "metallic"):
+        a_ = texels(d, m0[k + "_texture"])[1][..., m0[k + "_channel"]]
+        b_ = texels(d2, m1[k + "_texture"])[1][..., m1[k + "_channel"]]
+        assert np.array_equal(a_, b_), k
+    e0 = texels(d, d.materials[1]["emissive_texture"])[1][..., :3]
+    fmt, e1 = texels(d2, d2.materials[1]["emissive_texture"])
+    assert np.array_equal(np.asarray(e0, np.float32), np.asarray(e1[..., :3], np.float32))
+    d2.camera = dict(d.camera)
+    d2.sky = d.sky
+    a, b = oracle_mod.OracleScene(d), oracle_mod.OracleScene(d2)
+    ha, hb = a.primary_hits(), b.primary_hits()
+    for k in ("instance", "geometry", "primitive"):
+        assert np.array_equal(ha[k], hb[k])
+    np.testing.assert_allclose(a.render(frames=2), b.render(frames=2), atol=2e-5)
+
+
+def test_unrepresentable_emission_is_refused(tmp_path):
+    """A float emission texture whose texels are not sRGB bytes x one factor
+    per channel is refused rather than quantized."""
+    d = scenegen.textured_showcase(width=16, height=16)
+    w, h, fmt, data = d.textures[d.materials[1]["emissive_texture"]]
+    em = np.array(data, np.float32).reshape(h, w, 4)
+    em[0, 0, 0] = 1.2345   # between two sRGB steps of the 3.0 peak
+    d.textures[d.materials[1]["emissive_texture"]] = (w, h, fmt, em)
+    with pytest.raises(NotImplementedError):
+        scenegen.write_gltf(d, tmp_path / "bad.gltf")
+
+
 def test_errors_are_reported(tmp_path):
     with pytest.raises(runtime.HeliosError) as e:
         runtime.gltf_load(tmp_path / "missing.gltf", Capture())

@@ -1,7 +1,9 @@
 """GPU: a scene read with hh_scene_load_gltf renders like the scene it was
 exported from (scenegen.write_gltf): identical primary hits and radiance
 equal to float rounding (the albedo factor passes through v^(1/2.2) and
-back), on the wavefront path and in the output-buffer views."""
+back), on the wavefront path and in the output-buffer views; for the
+alpha-tested field and for the textured showcase (normal map, roughness and
+metallic channels, float emission texture, spot light)."""
 import numpy as np
 import pytest
 
@@ -11,14 +13,17 @@ from helios_amd.abi import OUTPUT_BUFFER_ALBEDO, OUTPUT_BUFFER_NORMALS
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("suffix", [".gltf", ".glb"])
-def test_loaded_scene_renders_like_the_original(gpu_ctx, tmp_path, suffix):
+@pytest.mark.parametrize("which,suffix", [("field", ".gltf"), ("field", ".glb"), ("textured", ".gltf")])
+def test_loaded_scene_renders_like_the_original(gpu_ctx, tmp_path, which, suffix):
     from helios_amd import runtime
-    d = scenegen.icosphere_field(seed=5, n_spheres=10, level=3, n_leaves=300, width=160, height=96,
-                                 max_ray_bounces=6)
+    if which == "field":
+        d = scenegen.icosphere_field(seed=5, n_spheres=10, level=3, n_leaves=300, width=160, height=96,
+                                     max_ray_bounces=6)
+    else:
+        d = scenegen.textured_showcase(width=128, height=96, max_ray_bounces=6)
     for l in d.lights:
         l["radius"] = 0.0                      # glTF lights are punctual
-    path = scenegen.write_gltf(d, tmp_path / ("field" + suffix))
+    path = scenegen.write_gltf(d, tmp_path / (which + suffix))
     a = runtime.Scene(gpu_ctx, d)
     b = runtime.Scene(gpu_ctx, None, commit=False)
     info = b.load_gltf(path)
