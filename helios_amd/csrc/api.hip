@@ -73,6 +73,16 @@ struct hh_context_t {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                       // shadow launches beside the next extension launch
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Pinned staging of a frame's scene table and view records (one DMA each,
+    // no pageable bounce buffer) and of the ray counters read back after it;
+    // ev_staged follows the last upload, so the host waits for it before it
+    // rewrites the staging. last_scenes: the scene table on the device, whose
+    // upload is skipped while it is unchanged.
+    unsigned char* h_stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t ev_staged = nullptr;
+    std::vector<SceneGPU> last_scenes;
+    unsigned long long* h_stats = nullptr;
     int num_cu = 0;
     Tuning tune;
     Wavefront wf;
@@ -392,6 +402,8 @@ hh_status hh_context_create(int device_ordinal, hh_context* out) {
     HH_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HH_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HH_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HH_HIP(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
+    HH_HIP(hipHostMalloc((void**)&c->h_stats, 2 * sizeof(unsigned long long)));
     HH_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device_ordinal));
     HH_HIP(hipMalloc(&c->d_coef, 30 * sizeof(float)));
     *out = c;
@@ -415,6 +427,9 @@ hh_status hh_context_destroy(hh_context ctx) {
     (void)hipStreamSynchronize(ctx->side);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_staged) (void)hipEventDestroy(ctx->ev_staged);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+    if (ctx->h_stats) (void)hipHostFree(ctx->h_stats);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1442,6 +1457,7 @@ static hipError_t ensure_tables(hh_context ctx, uint32_t nscenes, uint32_t nview
     if (nscenes > ctx->scene_table_cap) {
         if (ctx->d_scene_table) HH_TRY(hipFree(ctx->d_scene_table));
         ctx->d_scene_table = nullptr;
+        ctx->last_scenes.clear();
         ctx->drop_graphs();
         uint32_t cap = std::max<uint32_t>(nscenes, 64);
         HH_TRY(hipMalloc(&ctx->d_scene_table, cap * sizeof(SceneGPU)));
@@ -1469,6 +1485,35 @@ static hipError_t ensure_wavefront(hh_context ctx, uint32_t total) {
     return ctx->wf.ensure(total, qneed);
 }
 
+// The frame's scene table (skipped while the device copy is unchanged) and
+// view records, through the context's pinned staging.
+static hipError_t upload_tables(hh_context ctx, const std::vector<SceneGPU>& scenes, const std::vector<ViewGPU>& views) {
+    const size_t sb = scenes.size() * sizeof(SceneGPU), vb = views.size() * sizeof(ViewGPU);
+    const bool same = ctx->last_scenes.size() == scenes.size() &&
+                      std::memcmp(ctx->last_scenes.data(), scenes.data(), sb) == 0;
+    const size_t need = (same ? 0 : sb) + vb;
+    HH_TRY(hipEventSynchronize(ctx->ev_staged));   // the previous upload has left the staging
+    if (need > ctx->stage_cap) {
+        if (ctx->h_stage) HH_TRY(hipHostFree(ctx->h_stage));
+        ctx->h_stage = nullptr;
+        ctx->stage_cap = 0;
+        const size_t cap = std::max<size_t>(need, 16384);
+        HH_TRY(hipHostMalloc((void**)&ctx->h_stage, cap));
+        ctx->stage_cap = cap;
+    }
+    size_t off = 0;
+    if (!same) {
+        std::memcpy(ctx->h_stage, scenes.data(), sb);
+        HH_TRY(hipMemcpyAsync(ctx->d_scene_table, ctx->h_stage, sb, hipMemcpyHostToDevice, ctx->stream));
+        off = sb;
+    }
+    std::memcpy(ctx->h_stage + off, views.data(), vb);
+    HH_TRY(hipMemcpyAsync(ctx->d_views, ctx->h_stage + off, vb, hipMemcpyHostToDevice, ctx->stream));
+    HH_TRY(hipEventRecord(ctx->ev_staged, ctx->stream));
+    if (!same) ctx->last_scenes = scenes;
+    return hipSuccess;
+}
+
 static hh_status run_frame(hh_context ctx, const std::vector<SceneGPU>& scenes, const std::vector<ViewGPU>& views,
                            uint32_t total, uint32_t max_bounces, bool mega, const std::string& key_extra) {
     HH_HIP(ensure_tables(ctx, (uint32_t)scenes.size(), (uint32_t)views.size()));
@@ -1486,8 +1531,7 @@ static hh_status run_frame(hh_context ctx, const std::vector<SceneGPU>& scenes, 
         HH_HIP(ctx->wf.ensure_spill((size_t)ctx->num_cu * ctx->tune.blocks_per_cu * per_block));
     }
     if (ctx->wf.capacity != cap_before || ctx->wf.qcap != qcap_before || ctx->wf.spill != spill_before) ctx->drop_graphs();
-    HH_HIP(hipMemcpyAsync(ctx->d_scene_table, scenes.data(), scenes.size() * sizeof(SceneGPU), hipMemcpyHostToDevice, ctx->stream));
-    HH_HIP(hipMemcpyAsync(ctx->d_views, views.data(), views.size() * sizeof(ViewGPU), hipMemcpyHostToDevice, ctx->stream));
+    HH_HIP(upload_tables(ctx, scenes, views));
     LaunchCfg cfg{ctx->stream, ctx->num_cu, ctx->tune};
     cfg.side = ctx->side;
     cfg.fork = ctx->ev_fork;
@@ -1611,8 +1655,7 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
     // counters are not reallocated after they are cleared
     const uint32_t max_chunk = std::max<uint32_t>(1, std::min<uint32_t>(frames, fif));
     HH_HIP(ensure_wavefront(ctx, (uint32_t)(px * max_chunk)));
-    unsigned long long zero[2] = {0, 0};
-    HH_HIP(hipMemcpyAsync(ctx->wf.stats, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+    HH_HIP(hipMemsetAsync(ctx->wf.stats, 0, 2 * sizeof(unsigned long long), ctx->stream));
     const bool mega = (it->s.flags & HH_FLAG_MEGAKERNEL) != 0;
     std::vector<SceneGPU> scenes{scene->gpu};
     uint32_t launched = 0;
@@ -1658,11 +1701,10 @@ hh_status hh_integrator_render_frames(hh_integrator it, hh_scene scene, const hh
             it->tile_idx++;
         }
     }
-    unsigned long long stats[2];
-    HH_HIP(hipMemcpyAsync(stats, ctx->wf.stats, sizeof(stats), hipMemcpyDeviceToHost, ctx->stream));
+    HH_HIP(hipMemcpyAsync(ctx->h_stats, ctx->wf.stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HH_HIP(hipStreamSynchronize(ctx->stream));
-    it->ray_stats[0] = stats[0];
-    it->ray_stats[1] = stats[1];
+    it->ray_stats[0] = ctx->h_stats[0];
+    it->ray_stats[1] = ctx->h_stats[1];
     it->ray_stats[2] = launched;
     return HH_OK;
 }
