@@ -79,16 +79,35 @@ struct SlotInfo {
 // Path slots of a view are numbered pixel-major: the spp samples of one pixel
 // are consecutive, so the rays of a wave (64 consecutive slots of the first
 // queue) start from the same or neighbouring pixels and traverse the same
-// nodes. Results do not depend on the numbering.
+// nodes. The pixels follow in row-major order, except at one sample per
+// pixel on a tile of whole 8x8 blocks, where they follow in 8x8 blocks (in
+// row-major block order), so that a wave's 64 rays leave a square of pixels
+// instead of a row segment. Results do not depend on the numbering.
+__device__ inline bool block_order(const ViewGPU& v) { return v.spp == 1u && ((v.tile_w | v.tile_h) & 7u) == 0u; }
+// row-major tile pixel p -> its place in the slot order
+__device__ inline uint32_t pixel_order(const ViewGPU& v, uint32_t p) {
+    if (!block_order(v)) return p;
+    const uint32_t y = p / v.tile_w, x = p - y * v.tile_w;
+    return ((y >> 3) * (v.tile_w >> 3) + (x >> 3)) * 64u + (y & 7u) * 8u + (x & 7u);
+}
 __device__ inline SlotInfo slot_info(const ViewGPU* views, uint32_t nv, uint32_t slot) {
     SlotInfo si;
     si.view = nv == 1 ? 0u : find_view(views, nv, slot);
     const ViewGPU& v = views[si.view];
     uint32_t local = slot - v.slot_base;
-    uint32_t p = local / v.spp;
-    si.sample = local - p * v.spp;
-    si.y = v.tile_y + p / v.tile_w;
-    si.x = v.tile_x + p % v.tile_w;
+    uint32_t q = local / v.spp;
+    si.sample = local - q * v.spp;
+    uint32_t x, y;
+    if (block_order(v)) {
+        const uint32_t b = q >> 6, i = q & 63u, bw = v.tile_w >> 3;
+        x = (b % bw) * 8u + (i & 7u);
+        y = (b / bw) * 8u + (i >> 3);
+    } else {
+        y = q / v.tile_w;
+        x = q - y * v.tile_w;
+    }
+    si.x = v.tile_x + x;
+    si.y = v.tile_y + y;
     return si;
 }
 
@@ -853,13 +872,13 @@ __global__ void __launch_bounds__(kAccBlock) k_accumulate(const ViewGPU* __restr
             __syncthreads();  // the previous round's LDS reads are done
             for (uint32_t e = threadIdx.x; e < npb * cn; e += kAccBlock) {
                 const uint32_t pp = e / cn, ss = e - pp * cn;
-                stage[pp * (kAccChunk + 1) + ss] = L[v.slot_base + (p0 + pp) * spp + s0 + ss];
+                stage[pp * (kAccChunk + 1) + ss] = L[v.slot_base + pixel_order(v, p0 + pp) * spp + s0 + ss];
             }
             __syncthreads();
             if (!valid) continue;
             for (uint32_t k = 0; k < cn; k++) {
                 const uint32_t s = s0 + k;
-                const uint32_t slot = v.slot_base + p * spp + s;
+                const uint32_t slot = v.slot_base + pixel_order(v, p) * spp + s;
                 const float4 l = stage[threadIdx.x * (kAccChunk + 1) + k];
                 float3 c;
                 if (v.flags & HH_FLAG_NO_CLAMP_BIT) c = make_float3(l.x, l.y, l.z);
