@@ -1384,8 +1384,12 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     const bool side = pf == nullptr && cfg.side != nullptr && wf.spill_threads >= 2ull * grid * BS;
     uint32_t* const spill_side = side ? wf.spill + (size_t)grid * BS * kSpillStackDepth : wf.spill;
     bool joined = true;
+    hipError_t sync_err = hipSuccess;   // first failure of an event record or wait
+    auto keep = [&](hipError_t e) {
+        if (sync_err == hipSuccess) sync_err = e;
+    };
     auto join = [&]() {
-        if (!joined) (void)hipStreamWaitEvent(st, cfg.join, 0);
+        if (!joined) keep(hipStreamWaitEvent(st, cfg.join, 0));
         joined = true;
     };
     // one bounce of every path in the queue: extension rays, shading, shadow rays
@@ -1417,8 +1421,8 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
         }
         hipStream_t sst = st;
         if (side) {
-            (void)hipEventRecord(cfg.fork, st);
-            (void)hipStreamWaitEvent(cfg.side, cfg.fork, 0);
+            keep(hipEventRecord(cfg.fork, st));
+            keep(hipStreamWaitEvent(cfg.side, cfg.fork, 0));
             sst = cfg.side;
         }
         {
@@ -1429,7 +1433,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
                                                                wf.stats, tc, side ? spill_side : wf.spill);
         }
         if (side) {
-            (void)hipEventRecord(cfg.join, cfg.side);
+            keep(hipEventRecord(cfg.join, cfg.side));
             joined = false;
         }
     };
@@ -1464,6 +1468,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
         ProfScope ps(pf, PK_ACCUMULATE, st);
         k_accumulate<<<ag, kAccBlock, 0, st>>>(d_views, nv, wf.L, wf.feat, wf.capacity);
     }
+    if (sync_err != hipSuccess) return sync_err;
     return hipGetLastError();
 }
 
