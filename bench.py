@@ -70,7 +70,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-denoise", action="store_true")
     p.add_argument("--ref-frames", type=int, default=4096,
-                   help="frames of the plain accumulation that judges both methods")
+                   help="frames of the plain accumulation that scores both methods")
     p.add_argument("--weights", default=str(ROOT / "helios_amd" / "data" / "denoiser_unet.bin"))
     p.add_argument("--workload", default="single", choices=["single", "batch64", "combined4k"],
                    help="single: the 1080p scene (the headline metric); batch64: 64 uneven scenes/views in shared "

@@ -581,7 +581,7 @@ hh_status hh_bandwidth_probe(hh_context ctx, uint64_t bytes, uint32_t iters, dou
 
 /* ---------------------------------------------------------------------------
  * Learned denoiser (no counterpart in the reference, which only accumulates;
- * plain progressive accumulation stays the fallback and the judge of its
+ * plain progressive accumulation stays the fallback and the reference of its
  * error). A U-Net with levels of 32/48/64/96 channels: per level two 3x3
  * convolutions + ReLU, 2x2 average pooling down, nearest upsampling and skip
  * concatenation up, and a 1x1 head whose output is added to the input's

@@ -6,7 +6,7 @@
 //
 // It is a scalar transcription of the reference GLSL programs, templated on
 // the real type R: R = float is the FP32 transcription (the "reference path"
-// timed as the CPU baseline, and the bit-exact judge of primary-hit IDs);
+// timed as the CPU baseline, and the bit-exact reference of primary-hit IDs);
 // R = double is the FP64 converged oracle. Every function cites the shader or
 // host file:line it restates (paths relative to the reference root).
 //

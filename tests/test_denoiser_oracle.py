@@ -1,4 +1,4 @@
-"""CPU: pins the denoiser oracle (oracle/unet.py) before it judges the GPU.
+"""CPU: pins the denoiser oracle (oracle/unet.py) before it checks the GPU.
 
 * convolution: against a brute-force loop (correlation, zero padding, layout);
 * pooling / upsampling / convolution backward: adjoint identities;

@@ -89,7 +89,7 @@ def test_full_scene_against_oracle_at_reduced_resolution(gpu_ctx, oracle_mod, fu
 
 
 def test_full_scene_4096_frames_match_fp64_oracle_crop(gpu_ctx, oracle_mod, full_field, full_scene):
-    """The judge of the time-to-relMSE metric is the plain 4096-frame image
+    """The reference of the time-to-relMSE metric is the plain 4096-frame image
     at 1920x1080. Against the FP64 oracle on a 48x32 crop of the same frames
     (same per-pixel RNG streams; the oracle renders only the crop's pixels),
     it is within relMSE 1e-3."""
