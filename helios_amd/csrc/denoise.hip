@@ -57,8 +57,20 @@ __device__ inline bf16x8 zero8() {
 // tap, staging the tile cut a 1080p forward pass from 1.45 to 1.21 ms.)
 constexpr int kConvRows = 4;              // output rows per workgroup, one wave each
 constexpr int kConvThreads = 64 * kConvRows;
+// Waves per SIMD the LDS of one workgroup (halo tile + staged weights) allows
+// on a 160 KB CU, one wave per SIMD per workgroup: the register budget is
+// held to it, so registers never cost a workgroup that the LDS would admit.
+// (Wider outputs, which read their weights from global memory, keep the
+// compiler's own budget.)
+constexpr int conv_waves(int cout) {
+    if (cout > 96) return 1;
+    const int tile = (kConvRows + 2) * (16 * 4 + 2) * 4 * 16;
+    const int wts = 9 * cout * 5 * 16;
+    const int w = 163840 / (tile + wts);
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
 template <int C0, int UP0, int C1, int COUT, int EPI>
-__global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restrict__ src0, const __bf16* __restrict__ src1,
+__global__ void __launch_bounds__(kConvThreads) __attribute__((amdgpu_waves_per_eu(conv_waves(COUT)))) k_conv3x3(const __bf16* __restrict__ src0, const __bf16* __restrict__ src1,
                                                      const __bf16* __restrict__ wgt, const float* __restrict__ bias,
                                                      const __bf16* __restrict__ mask, void* __restrict__ out, uint32_t N,
                                                      uint32_t H, uint32_t W) {
@@ -73,7 +85,10 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
     static_assert(C0 % 8 == 0 && C1 % 8 == 0 && COUT % 16 == 0, "channel groups");
     constexpr int TILE = TR * TC * 4;
     constexpr int SPT = (TILE + kConvThreads - 1) / kConvThreads;  // staged 16-B chunks per thread
-    __shared__ bf16x8 tile[TILE];            // [row][col][4 chunks of 8 channels]
+    // [row][col][4 chunks of 8 channels], the chunks of pixel p stored in the
+    // order chunk ^ ((p >> 1) & 3), which spreads the fragment reads of
+    // neighbouring pixels over the banks
+    __shared__ bf16x8 tile[TILE];
     // Up to 48 output channels, the group's weights are staged in LDS too
     // ([tap][co] rows of 32 channels + 8 of padding, so the 16 rows a wave
     // reads at once fall on distinct banks), loaded with the tile.
@@ -141,7 +156,7 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
 #pragma unroll
         for (int i = 0; i < SPT; i++) {
             const int e = (int)threadIdx.x + i * kConvThreads;
-            if (e < TILE) tile[e] = stage[i];
+            if (e < TILE) tile[(e & ~3) | ((e & 3) ^ ((e >> 3) & 3))] = stage[i];
         }
         if constexpr (LDSW) {
 #pragma unroll
@@ -158,7 +173,10 @@ __global__ void __launch_bounds__(kConvThreads) k_conv3x3(const __bf16* __restri
             const int dy = tap / 3, dx = tap % 3;   // tile offsets (halo origin at -1)
             bf16x8 b[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) b[t] = tile[(((int)wv + dy) * TC + t * 16 + (int)r + dx) * 4 + (int)g];
+            for (int t = 0; t < NT; t++) {
+                const int pix = ((int)wv + dy) * TC + t * 16 + (int)r + dx;
+                b[t] = tile[pix * 4 + ((int)g ^ ((pix >> 1) & 3))];
+            }
             const int k = tap * CT + c;
 #pragma unroll
             for (int m = 0; m < MT; m++) {
