@@ -17,6 +17,9 @@
  *     host arrays); the library owns all device memory.
  *   - A context is externally synchronized: one host thread at a time.
  *   - Render calls are asynchronous on the context's HIP stream; hh_sync waits.
+ *     A frame may run part of its work on a second stream of the context; that
+ *     work is joined back into the context stream before the frame's last
+ *     launch, so a wait on the context stream covers the whole frame.
  *   - Surfaces are RGBA32F, row-major, row 0 = NDC y = -1 (the reference's
  *     storage-image orientation, renderer.cpp:396-399 flips it for display).
  */
