@@ -398,14 +398,23 @@ hh_status hh_context_create(int device_ordinal, hh_context* out) {
     hh_context c = new (std::nothrow) hh_context_t();
     if (!c) return fail(HH_ERR_OUT_OF_MEMORY, "context");
     c->device = device_ordinal;
-    HH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HH_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-    HH_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    HH_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-    HH_HIP(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
-    HH_HIP(hipHostMalloc((void**)&c->h_stats, 2 * sizeof(unsigned long long)));
-    HH_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device_ordinal));
-    HH_HIP(hipMalloc(&c->d_coef, 30 * sizeof(float)));
+    // a failure releases what was created so far
+    auto setup = [&]() -> hh_status {
+        HH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HH_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HH_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HH_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HH_HIP(hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming));
+        HH_HIP(hipHostMalloc((void**)&c->h_stats, 2 * sizeof(unsigned long long)));
+        HH_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device_ordinal));
+        HH_HIP(hipMalloc(&c->d_coef, 30 * sizeof(float)));
+        return HH_OK;
+    };
+    const hh_status st = setup();
+    if (st != HH_OK) {
+        (void)hh_context_destroy(c);
+        return st;
+    }
     *out = c;
     return HH_OK;
 }
@@ -413,7 +422,7 @@ hh_status hh_context_create(int device_ordinal, hh_context* out) {
 hh_status hh_context_destroy(hh_context ctx) {
     if (!ctx) return HH_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->drop_graphs();
     ctx->wf.release();
     ctx->bs.release();
@@ -424,14 +433,14 @@ hh_status hh_context_destroy(hh_context ctx) {
     if (ctx->prof.d_counts) (void)hipFree(ctx->prof.d_counts);
     if (ctx->prof.d_snaps) (void)hipFree(ctx->prof.d_snaps);
     for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
-    (void)hipStreamSynchronize(ctx->side);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->ev_staged) (void)hipEventDestroy(ctx->ev_staged);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->h_stats) (void)hipHostFree(ctx->h_stats);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HH_OK;
 }
