@@ -1,6 +1,7 @@
 """Turn rocprofv3 CSV output into the short summaries kept under profiles/.
 
   python -m helios_amd.profiling stats  <kernel_stats.csv> --cmd "<command>" --out profiles/<name>.md
+         [--trace <kernel_trace.csv> [--timeline <pass>]] [--title ...] [--note ...]
   python -m helios_amd.profiling traffic <fetch counter_collection.csv> <write counter_collection.csv>
          --kernel k_extend --cmd "<command>" --out profiles/extend_traffic.json
   python -m helios_amd.profiling mfma <kernel_stats.csv> <counter_collection.csv> --cmd "<commands>"
@@ -74,6 +75,30 @@ def passes_md(trace_csv, max_rows=12):
     for i, pz in enumerate(passes[:max_rows]):
         if pz["ext"]:
             lines.append(f"| {i + 1} | {pz['gen']} | {len(pz['ext'])} | {sum(pz['ext']) / len(pz['ext']) / 1e3:.1f} |")
+    return "\n".join(lines) + "\n"
+
+
+def timeline_md(trace_csv, pass_no):
+    """Every dispatch of wavefront pass `pass_no` (1-based, a pass starts at
+    each k_generate dispatch) with its hardware queue, start and end relative
+    to the pass's first dispatch, and duration: shows which launches ran side
+    by side (the shadow launch beside the next extension launch)."""
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if "k_generate<" in r["Kernel_Name"]]
+    if pass_no < 1 or pass_no > len(starts):
+        return ""
+    i0 = starts[pass_no - 1]
+    i1 = starts[pass_no] if pass_no < len(starts) else len(rows)
+    t0 = int(rows[i0]["Start_Timestamp"])
+    lines = ["", f"Pass {pass_no}, dispatch by dispatch (times in us from the pass's first dispatch):", "",
+             "| kernel | queue | start | end | us |", "|---|---|---|---|---|"]
+    last = t0
+    for r in rows[i0:i1]:
+        a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        lines.append(f"| {short(r['Kernel_Name'])} | {r['Queue_Id']} | {(a - t0) / 1e3:.1f} | {(b - t0) / 1e3:.1f} | "
+                     f"{(b - a) / 1e3:.1f} |")
+        last = max(last, b)
+    lines += ["", f"Pass {pass_no} spans {(last - t0) / 1e3:.1f} us."]
     return "\n".join(lines) + "\n"
 
 
@@ -220,6 +245,7 @@ def main():
     s.add_argument("--title", default="")
     s.add_argument("--trace", default=None, help="kernel_trace.csv of the same run: per-pass extension launches")
     s.add_argument("--note", default="")
+    s.add_argument("--timeline", type=int, default=0, help="with --trace: list every dispatch of this pass")
     s.add_argument("--out", required=True)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
@@ -246,6 +272,8 @@ def main():
         text = (f"# {a.title}\n\n" if a.title else "") + stats_md(a.csv, a.cmd)
         if a.trace:
             text += passes_md(a.trace)
+            if a.timeline:
+                text += timeline_md(a.trace, a.timeline)
         if a.note:
             text += "\n" + a.note + "\n"
         Path(a.out).write_text(text)
