@@ -249,6 +249,7 @@ constexpr uint32_t kStackFull = 0xFFFFFFFFu;  // trip count of a ray whose stack
 
 // Step shapes: if-if visits one node or leaf per step; while-while visits
 // inner nodes until the lane reaches a leaf (or finishes), then one leaf.
+constexpr uint32_t kLeafWait = 3;
 enum TravMode { kWhileWhile = 0, kIfIf = 1 };
 
 template <int L, bool ANY, bool COUNT>
@@ -408,10 +409,20 @@ struct Tracer {
     template <int MODE>
     __device__ inline bool step(const LaneStack& stk) {
         if (MODE == kWhileWhile) {
-            while ((cur & kLeafBit) == 0u && trips < kMaxTrips) {
-                trips++;
-                visit_inner(stk);
+            // inner nodes until this lane reaches a leaf, or until the lanes
+            // waiting at a leaf are kLeafWait times as many as those still in
+            // inner nodes (the wave then visits the waiting lanes' leaves and
+            // the others resume their inner nodes at the next step)
+            for (;;) {
+                const bool inner = (cur & kLeafBit) == 0u && trips < kMaxTrips;
+                const unsigned long long mi = __ballot(inner), mw = __ballot(!inner && cur != kDone);
+                if (mi == 0ull || (uint32_t)__popcll(mw) >= kLeafWait * (uint32_t)__popcll(mi)) break;
+                if (inner) {
+                    trips++;
+                    visit_inner(stk);
+                }
             }
+            if ((cur & kLeafBit) == 0u && trips < kMaxTrips) return false;
         } else if ((cur & kLeafBit) == 0u && trips < kMaxTrips) {
             trips++;
             visit_inner(stk);
