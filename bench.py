@@ -367,7 +367,9 @@ def bench_batch(a):
     build_s = time.perf_counter() - t0
     surfs = [runtime.Surface(ctx, d.width, d.height) for d in descs]
     views = [dict(scene=s, surface=f, camera=d.camera, max_ray_bounces=8) for s, f, d in zip(scenes, surfs, descs)]
-    tuning, tune_ms = runtime.autotune(ctx, views, frames=2, bvh_width=True)
+    # tuned on the frames the timed batch renders (16 per view): with fewer,
+    # frames-in-flight settings above that count would all look alike
+    tuning, tune_ms = runtime.autotune(ctx, views, frames=16, bvh_width=True)
     for v in views:
         v["spp"] = 16
     runtime.render_batch(ctx, views)          # warm-up (graphs)
