@@ -315,13 +315,14 @@ struct Tracer {
             sp = kLdsStack + kSpillStack;
         }
     }
+    // The LDS entry is read unconditionally (entry 0 when the stack is empty,
+    // the value then discarded): only the rare spill read needs a branch.
     __device__ inline void pop(const LaneStack& stk) {
-        if (sp == 0) {
-            cur = kDone;
-            return;
-        }
-        sp--;
-        cur = sp < (uint32_t)kLdsStack ? stk.lds[sp * stk.lds_stride] : stk.spill[(sp - kLdsStack) * stk.spill_stride];
+        const uint32_t s = sp == 0u ? 0u : sp - 1u;
+        uint32_t r = stk.lds[min(s, (uint32_t)kLdsStack - 1u) * stk.lds_stride];
+        if (s >= (uint32_t)kLdsStack) r = stk.spill[(s - kLdsStack) * stk.spill_stride];
+        cur = sp == 0u ? kDone : r;
+        sp = s;
     }
 
     // Children of inner node `cur` in front-to-back order: descend into the
@@ -331,10 +332,37 @@ struct Tracer {
         float tn[W];
         uint32_t ref[W];
         node_test<L>(nodes, cur, o, inv, tmin, h.t, tn, ref);
+        // With room for every child in the LDS part of the stack, children are
+        // stored without branches: each is written at the next free entry,
+        // which advances only when that child is hit (a missed child's entry
+        // is overwritten by the next one or left above the top).
+        const bool room = sp + (uint32_t)W <= (uint32_t)kLdsStack;
         if (ANY) {
             // any accepted hit ends the ray, so the order is free: children
             // in node order (measured to find occluders in fewer visits
-            // than nearest-first on the sun and point-light rays)
+            // than nearest-first on the sun and point-light rays). The
+            // lowest hit child is visited next, the others pushed above
+            // the stack's top in descending order.
+            if (room) {
+                uint32_t s = sp, next = kDone;
+#pragma unroll
+                for (int k = W - 1; k >= 1; k--) {
+                    stk.lds[s * stk.lds_stride] = ref[k];
+                    const bool hit = tn[k] != INFINITY;
+                    s += hit ? 1u : 0u;
+                    next = hit ? ref[k] : next;
+                }
+                if (tn[0] != INFINITY) {
+                    cur = ref[0];
+                    sp = s;
+                } else if (s != sp) {
+                    cur = next;
+                    sp = s - 1u;
+                } else {
+                    pop(stk);
+                }
+                return;
+            }
 #pragma unroll
             for (int k = W - 1; k >= 0; k--)
                 if (tn[k] != INFINITY) push(stk, ref[k]);
@@ -364,13 +392,20 @@ struct Tracer {
             pop(stk);
             return;
         }
-        // (a branch-free variant that stores every child, sending unused
-        // ones to a discard slot, measured slower: it raised register
-        // pressure past the kernels' wave budget)
+        if (room) {
+            uint32_t s = sp;
 #pragma unroll
-        for (int k = W - 1; k >= 1; k--)
-            if (tn[k] != INFINITY) push(stk, ref[k]);
-        check_full();
+            for (int k = W - 1; k >= 1; k--) {
+                stk.lds[s * stk.lds_stride] = ref[k];
+                s += tn[k] != INFINITY ? 1u : 0u;
+            }
+            sp = s;
+        } else {
+#pragma unroll
+            for (int k = W - 1; k >= 1; k--)
+                if (tn[k] != INFINITY) push(stk, ref[k]);
+            check_full();
+        }
         cur = ref[0];
     }
 
