@@ -179,6 +179,22 @@ __device__ inline float hit_alpha(const SceneGPU& S, uint32_t inst, uint32_t geo
     return sample_tex(S, slot, tu, tv).w;
 }
 
+// Reciprocal direction for the box tests: v_rcp_f32 (within 1 ulp of 1/d,
+// +-inf for +-0) instead of a correctly rounded division (about ten
+// instructions each). The triangle test does not use it.
+__device__ inline f3 box_inv(f3 d) {
+    return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+}
+
+// Padding of the far distance in the slab test. Each slab distance
+// fl(fl(lo - o) * rcp(d)) carries at most 4u of relative error (u = 2^-24:
+// one rounding each for the subtraction and the product, 2u for the
+// reciprocal), so a computed entry exceeds the computed exit of a box the
+// exact ray meets by at most (1 + 4u) / (1 - 4u), and the product with the
+// padding rounds by one more u: 1 + 10u covers it (Ize's gamma_n argument with
+// n = 4), so a box is never culled that the exact ray enters.
+constexpr float kBoxPad = 0x1.00000ap+0f;
+
 // One slab test in the robust form (Ize 2013): returns the entry distance or
 // +inf on a miss. min/max drop a NaN operand (0 * inf for zero directions).
 __device__ inline float slab(float lox, float hix, float loy, float hiy, float loz, float hiz, f3 o, f3 inv, float tmin,
@@ -188,7 +204,7 @@ __device__ inline float slab(float lox, float hix, float loy, float hiy, float l
     float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
     float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
     float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-    return a <= b * 1.0000004f ? a : INFINITY;
+    return a <= b * kBoxPad ? a : INFINITY;
 }
 
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
@@ -280,7 +296,7 @@ struct Tracer {
         insts = (const HH_GLOBAL InstTrav*)scene.inst_trav;
         wo = ro;
         wd = rd;
-        winv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        winv = box_inv(rd);
         o = wo;
         inv = winv;
         wt = wt_pre(rd);
@@ -480,7 +496,7 @@ struct Tracer {
             if (COUNT) cnt->insts++;
             f3 od;
             xform_ray(it.w2o, wo, wd, o, od);
-            inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+            inv = box_inv(od);
             wt = wt_pre(od);
             push(stk, kMarker);
             check_full();
@@ -526,7 +542,7 @@ __device__ inline void trace_packet(const SceneGPU& S, bool active, f3 wo, f3 wd
     const HH_GLOBAL void* nodes = (const HH_GLOBAL void*)S.nodes;
     const HH_GLOBAL float4* tris = (const HH_GLOBAL float4*)S.tris;
     const HH_GLOBAL InstTrav* insts = (const HH_GLOBAL InstTrav*)S.inst_trav;
-    const f3 winv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+    const f3 winv = box_inv(wd);
     f3 o = wo, inv = winv;
     WT wt = wt_pre(wd);
     const bool alpha_on = (rflags & 1u) == 0u;
@@ -604,7 +620,7 @@ __device__ inline void trace_packet(const SceneGPU& S, bool active, f3 wo, f3 wd
             const HH_GLOBAL InstTrav& it = insts[inst];
             f3 od;
             xform_ray(it.w2o, wo, wd, o, od);
-            inv = mk(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+            inv = box_inv(od);
             wt = wt_pre(od);
             push(kMarker);
             cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)it.blas_root);
