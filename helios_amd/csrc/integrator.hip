@@ -543,26 +543,26 @@ struct QueueCursor {
     __device__ inline QueueCursor(const uint32_t* count_, uint32_t* head_, uint32_t seg_, uint32_t refill_)
         : count(count_), head(head_), seg(seg_), refill(refill_), shard(wave_shard()), tried(0u), drained(false) {}
     // Queue entry for this lane (kNoEntry: none). Called by the whole wave in
-    // converged control flow, so lane 0 is active for the atomics.
+    // converged control flow.
     __device__ inline uint32_t next(bool idle_lane) {
         if (drained) return kNoEntry;
         const unsigned long long idle = __ballot(idle_lane);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (nidle < refill && nidle != 64u) return kNoEntry;
-        const uint32_t lane = __lane_id();
-        const unsigned long long below = lane == 0 ? 0ull : (idle & ((1ull << lane) - 1ull));
-        const uint32_t rank = (uint32_t)__popcll(below);
+        // idle lanes below this one (mbcnt: no lane mask or lane index kept
+        // in registers); the lowest idle lane (rank 0) claims for the wave
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
         uint32_t got = kNoEntry, done = 0u;
         while (done < nidle) {
             const uint32_t want = nidle - done;
             const uint32_t n = count[shard * kCtrStride];
             uint32_t base = 0;
-            if (lane == 0) {
+            if (idle_lane && rank == 0u) {
                 // a segment already read to its end costs no atomic
                 base = __hip_atomic_load(head + shard * kCtrStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (base < n) base = atomicAdd(head + shard * kCtrStride, want);
             }
-            base = __builtin_amdgcn_readfirstlane(base);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)__builtin_ctzll(idle));
             const uint32_t avail = base < n ? min(n - base, want) : 0u;
             if (idle_lane && rank >= done && rank < done + avail) got = shard * seg + base + (rank - done);
             done += avail;

@@ -455,6 +455,15 @@ struct Tracer {
         return false;
     }
 
+    // The lane is at an inner node with visits left: (cur & kLeafBit) == 0 and
+    // trips < kMaxTrips = 2^22, tested as one sign compare. trips never
+    // exceeds kMaxTrips except as kStackFull, whose shifted value also has
+    // bit 31 set.
+    __device__ inline bool at_inner() const {
+        static_assert(kLeafBit == 0x80000000u && kMaxTrips == (1u << 22), "one-compare inner test");
+        return (int32_t)(cur | (trips << 9)) >= 0;
+    }
+
     // Advances the traversal; returns true once the ray is finished (h holds
     // the result).
     template <int MODE>
@@ -463,18 +472,23 @@ struct Tracer {
             // inner nodes until this lane reaches a leaf, or until the lanes
             // waiting at a leaf are kLeafWait times as many as those still in
             // inner nodes (the wave then visits the waiting lanes' leaves and
-            // the others resume their inner nodes at the next step)
-            for (;;) {
-                const bool inner = (cur & kLeafBit) == 0u && trips < kMaxTrips;
-                const unsigned long long mi = __ballot(inner), mw = __ballot(!inner && cur != kDone);
-                if (mi == 0ull || (uint32_t)__popcll(mw) >= kLeafWait * (uint32_t)__popcll(mi)) break;
-                if (inner) {
+            // the others resume their inner nodes at the next step). Live
+            // lanes (ml) are inner or waiting, so the loop runs while
+            // live < (kLeafWait + 1) * inner. The masks come straight from
+            // one compare each and the inner lanes are selected by their
+            // mask bit, so no per-lane flag is kept across the loop's blocks.
+            unsigned long long mi = __builtin_amdgcn_ballot_w64(at_inner());
+            unsigned long long ml = __builtin_amdgcn_ballot_w64(cur != kDone);
+            while (mi != 0ull && (uint32_t)__popcll(ml) < (kLeafWait + 1u) * (uint32_t)__popcll(mi)) {
+                if (__builtin_amdgcn_inverse_ballot_w64(mi)) {
                     trips++;
                     visit_inner(stk);
                 }
+                mi = __builtin_amdgcn_ballot_w64(at_inner());
+                ml = __builtin_amdgcn_ballot_w64(cur != kDone);
             }
-            if ((cur & kLeafBit) == 0u && trips < kMaxTrips) return false;
-        } else if ((cur & kLeafBit) == 0u && trips < kMaxTrips) {
+            if (at_inner()) return false;
+        } else if (at_inner()) {
             trips++;
             visit_inner(stk);
             return false;
