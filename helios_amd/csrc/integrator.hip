@@ -1374,9 +1374,14 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
     const uint32_t seg = (uint32_t)seg64;
     // camera rays in packets when all views share one scene (the counting
     // variant keeps the per-lane traversal, whose fetches it counts)
-    bool packets = !COUNT;
-    for (uint32_t i = 0; i < nv; i++)
+    bool packets = !COUNT, one_scene = true;
+    for (uint32_t i = 0; i < nv; i++) {
+        one_scene = one_scene && h_views[i].scene == h_views[0].scene;
         packets = packets && h_views[i].scene == h_views[0].scene && h_views[i].spp >= kPacketMinSpp;
+    }
+    // the traversal kernels read a view only for its scene: with one scene
+    // they take view 0 and skip the per-ray search of the view table
+    const uint32_t nv_trace = one_scene ? 1u : nv;
     // Shadow launches on the side stream, beside the next extension launch
     // (neither reads what the other writes); `join` makes the main stream
     // wait for the last one before the next reset, shade, continuation or
@@ -1402,7 +1407,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
                                                         wf.stats);
         } else {
             ProfScope ps(pf, PK_EXTEND, st);
-            k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
+            k_extend<W, BS, MODE, COUNT><<<grid, BS, 0, st>>>(d_scenes, d_views, nv_trace, wf.q_o[a], wf.q_d[a], wf.q_slot[a],
                                                               ctr + ctr_index(kCtrExt0 + a, 0),
                                                               ctr + ctr_index(kCtrExtHead, 0), seg,
                                                               d == 0 ? refill_camera : refill,
@@ -1427,7 +1432,7 @@ static hipError_t frame_impl(const LaunchCfg& cfg, const SceneGPU* d_scenes, con
         }
         {
             ProfScope ps(pf, PK_SHADOW, sst);
-            k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, sst>>>(d_scenes, d_views, nv, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
+            k_shadow<W, BS, MODE, COUNT><<<grid, BS, 0, sst>>>(d_scenes, d_views, nv_trace, wf.s_o, wf.s_d, wf.s_c, wf.s_slot,
                                                                ctr + ctr_index(kCtrShadow, 0),
                                                                ctr + ctr_index(kCtrShadowHead, 0), seg, refill, wf.L,
                                                                wf.stats, tc, side ? spill_side : wf.spill);
