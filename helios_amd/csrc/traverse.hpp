@@ -273,10 +273,10 @@ struct Tracer {
     static constexpr int W = layout_width(L);
     const SceneGPU* S;
     // scene arrays read on every visit, held in registers: reading them
-    // through S would put a dependent load in front of every node fetch
+    // through S would put a dependent load in front of every node fetch (the
+    // instance records, read once per instance entry, are reached through S)
     const HH_GLOBAL void* nodes;
     const HH_GLOBAL float4* tris;
-    const HH_GLOBAL InstTrav* insts;
     f3 wo, wd, winv;   // world-space ray
     f3 o, inv;         // ray in the current (world or instance) space
     WT wt;
@@ -293,7 +293,6 @@ struct Tracer {
         S = &scene;
         nodes = (const HH_GLOBAL void*)scene.nodes;
         tris = (const HH_GLOBAL float4*)scene.tris;
-        insts = (const HH_GLOBAL InstTrav*)scene.inst_trav;
         wo = ro;
         wd = rd;
         winv = box_inv(rd);
@@ -506,7 +505,7 @@ struct Tracer {
             pop(stk);
         } else if (inst < 0) {
             inst = (int)leaf_start(cur);
-            const HH_GLOBAL InstTrav& it = insts[inst];
+            const HH_GLOBAL InstTrav& it = ((const HH_GLOBAL InstTrav*)S->inst_trav)[inst];
             if (COUNT) cnt->insts++;
             f3 od;
             xform_ray(it.w2o, wo, wd, o, od);
