@@ -157,11 +157,11 @@ def roofline(ctx, it, scene, desc, surf, chunk, node_bytes, a):
     acceleration structure (every BLAS/TLAS node, leaf triangle and instance
     record once): what the launch must move from HBM with ideal caching.
     traffic = HBM bytes per launch measured by FETCH_SIZE / WRITE_SIZE at the
-    same shape (profiles/extend_traffic.json), scaled by the copy-kernel
+    same shape (profiles/extend_traffic*.json), scaled by the copy-kernel
     calibration. The per-lane fetch bytes (every lane's node, triangle and
     instance fetches) are served mostly by L1/L2 and are reported beside it,
     as is the issue bound the kernel actually runs into
-    (profiles/extend_issue.json)."""
+    (profiles/extend_issue*.json)."""
     ctx.set_profiling(timing=True)
     ctx.reset_profile()
     it.render(scene, desc.camera, surf, frames=chunk)
@@ -188,12 +188,13 @@ def roofline(ctx, it, scene, desc, surf, chunk, node_bytes, a):
                                     per_ray["insts"] * INST_BYTES + RAY_QUEUE_BYTES)
     shape = {"frames_in_flight": chunk, "continuation_depth": a.continuation_depth, "bvh_builder": a.bvh_builder}
     traffic, l2, issue = None, None, None
-    tfile, ifile = ROOT / "profiles" / "extend_traffic.json", ROOT / "profiles" / "extend_issue.json"
-    if tfile.exists():
+    # counter profiles exist per launch shape (extend_traffic*.json,
+    # extend_issue*.json); the one taken at this run's shape is used
+    for tfile in sorted((ROOT / "profiles").glob("extend_traffic*.json")):
         tj = json.loads(tfile.read_text())
         if tj.get("shape") == shape:
             traffic, l2 = tj.get("hbm_bytes_per_launch"), tj.get("l2")
-    if ifile.exists():
+    for ifile in sorted((ROOT / "profiles").glob("extend_issue*.json")):
         ij = json.loads(ifile.read_text())
         if ij.get("shape") == shape:
             issue = {k: ij["kernels"][k] for k in ij.get("kernels", {})}
